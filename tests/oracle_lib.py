@@ -1,0 +1,160 @@
+"""ctypes access to the oracle (own CPU restatement) and, where it was built,
+oracle/_ref (the reference's host numerics). Test infrastructure only."""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle_sie.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libsie_ref.so")
+
+_P = ctypes.c_void_p
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_P)
+
+
+class Lib:
+    """Uniform wrapper: prefix 'oracle_' or 'ref_'."""
+
+    def __init__(self, path, prefix):
+        self.lib = ctypes.CDLL(path)
+        self.prefix = prefix
+
+    def f(self, name):
+        return getattr(self.lib, self.prefix + name)
+
+    def pair_regular(self, tt, ti, st, si, rule, k1, k2, singular):
+        out = np.zeros(72)
+        fn = self.f("pair_regular")
+        fn.argtypes = [_P, _P, _P, _P, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, _P]
+        fn(_ptr(np.ascontiguousarray(tt, float)), _ptr(np.ascontiguousarray(ti, np.int32)),
+           _ptr(np.ascontiguousarray(st, float)), _ptr(np.ascontiguousarray(si, np.int32)),
+           rule, k1, k2, int(singular), _ptr(out))
+        return out.view(complex).reshape(4, 3, 3)
+
+    def pair_singular(self, tt, ti, st, si, rule, k1, k2):
+        out = np.zeros(72)
+        fn = self.f("pair_singular")
+        fn.argtypes = [_P, _P, _P, _P, ctypes.c_int, ctypes.c_double, ctypes.c_double, _P]
+        fn(_ptr(np.ascontiguousarray(tt, float)), _ptr(np.ascontiguousarray(ti, np.int32)),
+           _ptr(np.ascontiguousarray(st, float)), _ptr(np.ascontiguousarray(si, np.int32)),
+           rule, k1, k2, _ptr(out))
+        return out.view(complex).reshape(4, 3, 3)
+
+    def wilton(self, r, tri):
+        out = np.zeros(12)
+        fn = self.f("wilton")
+        fn.argtypes = [_P, _P, _P]
+        fn(_ptr(np.ascontiguousarray(r, float)), _ptr(np.ascontiguousarray(tri, float)), _ptr(out))
+        return out
+
+    def mass_batch(self, tri, idx, rule, T):
+        out = np.zeros(144)
+        fn = self.f("mass_batch")
+        fn.argtypes = [_P, _P, ctypes.c_int, _P, _P]
+        T = np.ascontiguousarray(T, complex)
+        fn(_ptr(np.ascontiguousarray(tri, float)), _ptr(np.ascontiguousarray(idx, np.int32)), rule,
+           _ptr(T), _ptr(out))
+        return out.view(complex).reshape(8, 3, 3)
+
+    def excitation_planewave(self, tri, idx, rule, E0, kh, k, eta):
+        ve = np.zeros(6)
+        vh = np.zeros(6)
+        fn = self.f("excitation_planewave")
+        fn.argtypes = [_P, _P, ctypes.c_int, _P, _P, ctypes.c_double, ctypes.c_double, _P, _P]
+        fn(_ptr(np.ascontiguousarray(tri, float)), _ptr(np.ascontiguousarray(idx, np.int32)), rule,
+           _ptr(np.ascontiguousarray(E0, float)), _ptr(np.ascontiguousarray(kh, float)), k, eta,
+           _ptr(ve), _ptr(vh))
+        return ve.view(complex), vh.view(complex)
+
+    def face_basis(self, tri, basis):
+        u = np.zeros(3)
+        v = np.zeros(3)
+        fn = self.f("face_basis")
+        fn.argtypes = [_P, ctypes.c_int, _P, _P]
+        fn(_ptr(np.ascontiguousarray(tri, float)), basis, _ptr(u), _ptr(v))
+        return u, v
+
+    def rule(self, r):
+        if self.prefix == "oracle_":
+            n = self.lib.oracle_rule_order(r)
+            xy = np.zeros(2 * n, np.float32)
+            w = np.zeros(n, np.float32)
+            self.lib.oracle_rule(r, _ptr(xy), _ptr(w))
+        else:
+            n = self.lib.ref_dunavant_order_num(r)
+            xy = np.zeros(2 * n, np.float32)
+            w = np.zeros(n, np.float32)
+            self.lib.ref_make_quadrature(r, _ptr(xy), _ptr(w))
+        return xy.reshape(-1, 2), w
+
+
+def oracle():
+    return Lib(ORACLE_SO, "oracle_")
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    return Lib(REF_SO, "ref_")
+
+
+def icosphere(level, radius=1.0):
+    """Oracle icosphere (restatement of SphericalSurface): V, F, E, FE."""
+    lib = ctypes.CDLL(ORACLE_SO)
+    counts = np.zeros(3, np.int32)
+    lib.oracle_icosphere.argtypes = [ctypes.c_float, ctypes.c_int, _P, _P, _P, _P, _P]
+    lib.oracle_icosphere(radius, level, None, None, None, None, _ptr(counts))
+    nv, nf, ne = (int(c) for c in counts)
+    V = np.zeros((nv, 3))
+    F = np.zeros((nf, 3), np.int32)
+    E = np.zeros((ne, 2), np.int32)
+    FE = np.zeros((nf, 3), np.int32)
+    lib.oracle_icosphere(radius, level, _ptr(V), _ptr(F), _ptr(E), _ptr(FE), _ptr(counts))
+    return V, F, E, FE
+
+
+def ref_icosphere(level, radius=1.0):
+    lib = ctypes.CDLL(REF_SO)
+    nv, nf, ne = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    lib.ref_icosphere_counts(level, ctypes.byref(nv), ctypes.byref(nf), ctypes.byref(ne))
+    V = np.zeros((nv.value, 3))
+    F = np.zeros((nf.value, 3), np.int32)
+    E = np.zeros((ne.value, 2), np.int32)
+    FE = np.zeros((nf.value, 3), np.int32)
+    lib.ref_icosphere.argtypes = [ctypes.c_float, ctypes.c_int, _P, _P, _P, _P]
+    lib.ref_icosphere(radius, level, _ptr(V), _ptr(F), _ptr(E), _ptr(FE))
+    return V, F, E, FE
+
+
+F32 = np.float32
+
+
+def assemble_gstc(lib_path, prefix, mesh, rule, phys, tensors, uniform, basis, with_calderon=True,
+                  with_mass=True):
+    """phys: dict of float32-castable k1,k2,mu_r1,mu_r2,eps_r1,eps_r2,mu0,eps0,omega,thr."""
+    V, F, E, FE = mesh
+    nf, ne = len(F), len(E)
+    dim = 4 * ne
+    Z = np.zeros((dim, dim), complex, order="F")
+    lib = ctypes.CDLL(lib_path)
+    fl = [F32(phys[k]) for k in ("k1", "k2", "mu_r1", "mu_r2", "eps_r1", "eps_r2", "mu0", "eps0", "omega", "thr")]
+    t = np.ascontiguousarray(tensors, complex)
+    if prefix == "oracle_":
+        fn = lib.oracle_assemble_gstc
+        fn.argtypes = [_P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int] + [ctypes.c_float] * 10 + \
+            [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P]
+        fn(_ptr(np.ascontiguousarray(V)), _ptr(np.ascontiguousarray(F)), _ptr(np.ascontiguousarray(FE)),
+           nf, ne, rule, *fl, _ptr(t), int(uniform), basis, int(with_calderon), int(with_mass), _ptr(Z))
+    else:
+        fn = lib.ref_assemble_gstc
+        fn.argtypes = [_P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int] + [ctypes.c_float] * 10 + \
+            [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P]
+        fn(_ptr(np.ascontiguousarray(V)), _ptr(np.ascontiguousarray(F)), _ptr(np.ascontiguousarray(FE)),
+           nf, ne, rule, *fl, _ptr(t), int(uniform), basis, int(with_mass), _ptr(Z))
+    return Z
