@@ -1,0 +1,132 @@
+/*
+ * sie_capi.h — C ABI of libsie_amd (MI355X / gfx950), the SIE-MoM hot path:
+ * RWG impedance-matrix assembly (T-EFIE, T-PMCHWT, SIE-GSTC), dense complex LU
+ * with triangular solves, FP64 iterative refinement, GMRES and the adjoint
+ * gradient of the GSTC susceptibilities.
+ *
+ * Conventions
+ *  - Complex numbers are interleaved (re, im) doubles or floats.
+ *  - Matrices are column-major with a leading dimension, as in the reference
+ *    (columnMajorIndex, include/sie_mom/cuda/sie_kernels_assembly.cuh:12-14).
+ *  - Unknowns are ordered block-major by RWG edge id: [J1|M1|J2|M2] for GSTC
+ *    (src/sie_mom/solvers/gstc_sie.cu:122), [J|M] for PMCHWT, [J] for EFIE.
+ *  - Pointers named *_dev are device pointers owned by the caller; everything
+ *    else is host memory. `stream` is a hipStream_t (NULL = default stream);
+ *    every call is stream-ordered and returns once the work is enqueued unless
+ *    it documents a host synchronisation.
+ *  - Every function returns a status: SIE_OK or one of the SIE_ERR_* codes,
+ *    with a message readable through sie_last_error() on the calling thread.
+ *    The codes mirror the reference's exception classes: invalid_argument,
+ *    overflow_error, and runtime_error for singular / non-finite / device
+ *    failures (gstc_sie.cu:71-179, dense_lu_solver.cuh:102-111).
+ */
+#ifndef SIE_CAPI_H
+#define SIE_CAPI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SIE_OK 0
+#define SIE_ERR_INVALID 1
+#define SIE_ERR_OVERFLOW 2
+#define SIE_ERR_SINGULAR 3
+#define SIE_ERR_NONFINITE 4
+#define SIE_ERR_DEVICE 5
+
+/* Message of the last failed call on this thread ("" after a success). */
+const char* sie_last_error(void);
+
+/* ------------------------------------------------------------------ mesh */
+/* A closed, outward-oriented triangle mesh uploaded to the device once.
+ * Replaces mesh::MeshData + copy_problem_to_device (include/sie_mom/mesh/
+ * spherical_surface.hpp:10-15, src/sie_mom/solvers/gstc_sie.cu:181-242).
+ * vertices: nv x 3 doubles; faces, face_edges: nf x 3 ints (local edge j runs
+ * vertex j -> j+1); ne: number of RWG edges. Every edge must be shared by
+ * exactly two faces. */
+typedef struct sie_mesh* sie_mesh_t;
+int sie_mesh_create(const double* vertices, int nv, const int* faces, int nf,
+                    const int* face_edges, int ne, sie_mesh_t* out);
+int sie_mesh_destroy(sie_mesh_t mesh);
+
+/* Mesh generators. Call once with NULL arrays to get counts = {nv, nf, ne},
+ * then again with arrays of those sizes (vertices nv*3, faces nf*3, edges ne*2,
+ * face_edges nf*3). Edge ids follow the reference's first-encounter order
+ * (src/sie_mom/mesh/spherical_surface.cpp:100-135).
+ * sie_icosphere replaces mesh::SphericalSurface(radius, level)
+ * (spherical_surface.cpp:19-98). sie_spindle is the circular-arc spindle of
+ * the reference's gmsh CircularOgive (src/sie_mom/mesh/circular_ogive.py:13-80):
+ * the revolution of an arc of radius arc_radius about its chord (length chord). */
+int sie_icosphere(float radius, int level, double* vertices, int* faces, int* edges,
+                  int* face_edges, int* counts);
+int sie_spindle(double chord, double arc_radius, int n_rings, double* vertices, int* faces,
+                int* edges, int* face_edges, int* counts);
+
+/* Launch parameters of the reference assembly kernels, already rounded to
+ * float exactly as the reference casts them (gstc_sie.cu:281-290). */
+typedef struct {
+    int rule;         /* Dunavant rule 1..20 */
+    float k1, k2;     /* interior / exterior wavenumber */
+    float mu_r1, mu_r2, eps_r1, eps_r2;
+    float mu0, eps0, omega;
+    float threshold;  /* near-pair centroid distance (0.1 lambda0 by default) */
+} sie_calderon_params;
+
+/* Raw Calderon quadrature of every face pair, no epilogue: out_dev holds
+ * nf*nf*72 doubles, pair (m,n) at (m*nf+n)*72 as 9 complex TL(k1), TL(k2),
+ * TK(k1), TK(k2). The device counterpart of calculateDoubleQuadrature +
+ * calculateSingularPartQuadrature per pair (include/sie_mom/cuda/
+ * dunavant_quadrature.cuh:207-324, 769-875); for validation on small meshes. */
+int sie_calderon_pair_blocks(sie_mesh_t mesh, const sie_calderon_params* p,
+                             double* out_dev, void* stream);
+
+#define SIE_ASM_CALDERON 1
+#define SIE_ASM_MASS 2
+/* eta0-scaled SIE-GSTC matrix D A D (dim 4E), overwriting Z_dev.
+ * Replaces assemble_system_matrix (gstc_sie.cu:244-317): the Calderon kernel
+ * assemble_Z_calderon_eta0_scaled_kernel (sie_kernels_assembly.cuh:156-272)
+ * and the mass kernel assemble_Z_mass_eta0_scaled_kernel (:460-594).
+ * tensors_dev: 32 complex per face (or 32 if uniform): the 4x4 row-major
+ * fields-to-currents map followed by its inverse (gstc_sie.hpp:205-213).
+ * basis: 0 = face frame, 1 = spherical frame. flags: SIE_ASM_* (both = full). */
+int sie_assemble_gstc(sie_mesh_t mesh, const sie_calderon_params* p, const double* tensors_dev,
+                      int uniform, int basis, int flags, double* Z_dev, long long ldz,
+                      void* stream);
+
+/* PEC T-EFIE matrix jw mu2 TL(k2), dim E (assemble_pec_efie_operator_kernel,
+ * sie_kernels_assembly.cuh:610-705, launched with k2 twice by pec_solver.cu:352-359). */
+int sie_assemble_pec(sie_mesh_t mesh, const sie_calderon_params* p, double* Z_dev,
+                     long long ldz, void* stream);
+
+/* T-PMCHWT matrix, dim 2E, eta0 scaled (t_pmchwt_kernel,
+ * src/sie_mom/solvers/dielectric/dielectric_kernels.cu:23-150). */
+int sie_assemble_pmchwt(sie_mesh_t mesh, const sie_calderon_params* p, double* Z_dev,
+                        long long ldz, void* stream);
+
+/* Plane wave bound to its medium (cuda::PlaneWave, include/sie_mom/excitation/
+ * source_functors.cuh:14-42; region 1 interior, 2 exterior). */
+typedef struct {
+    double E0[3];
+    double khat[3];
+    double k, eta;
+    int region;
+} sie_plane_wave;
+
+#define SIE_RHS_GSTC 0
+#define SIE_RHS_PMCHWT 1
+#define SIE_RHS_EFIE 2
+/* Tested incident fields of nwaves plane waves, one RHS column each, written
+ * (not accumulated) to rhs_dev (leading dimension ldb, complex).
+ * GSTC: assemble_V_vector_eta0_scaled_kernel (sie_kernels_utils.cuh:185-232),
+ * using eta0 = sqrt(double(mu0)/double(eps0)) on the M rows;
+ * PMCHWT: assemble_dielectric_rhs_kernel with sign +1 exterior / -1 interior
+ * and sign*sqrt(vac_mu/vac_eps) on M rows (dielectric_solver.cu:400-531);
+ * EFIE: assemble_pec_rhs_kernel, +<f, E_inc> (pec_solver.cu:243-264). */
+int sie_rhs_plane_waves(sie_mesh_t mesh, int kind, int rule, const sie_plane_wave* waves,
+                        int nwaves, float mu0, float eps0, double vac_mu, double vac_eps,
+                        double* rhs_dev, long long ldb, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SIE_CAPI_H */

@@ -1,0 +1,87 @@
+"""ctypes binding of libsie_amd.so, the C ABI declared in include/sie_mom/sie_capi.h.
+
+The library is built in place by ``sie-gstc-2026_amd/build.sh`` (or
+``__graft_entry__.build()``). There is no fallback: if the shared object is
+missing, importing this module raises.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsie_amd.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"{LIB_PATH} is missing: build it with sie-gstc-2026_amd/build.sh (no CPU fallback exists)"
+    )
+
+lib = ctypes.CDLL(LIB_PATH)
+
+P = ctypes.c_void_p
+I = ctypes.c_int
+LL = ctypes.c_longlong
+D = ctypes.c_double
+Fl = ctypes.c_float
+
+SIE_OK = 0
+SIE_ERR_INVALID = 1
+SIE_ERR_OVERFLOW = 2
+SIE_ERR_SINGULAR = 3
+SIE_ERR_NONFINITE = 4
+SIE_ERR_DEVICE = 5
+
+
+class SieError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+class SieInvalidArgument(SieError, ValueError):
+    pass
+
+
+class SieOverflow(SieError, OverflowError):
+    pass
+
+
+def check(code):
+    if code == SIE_OK:
+        return
+    msg = lib.sie_last_error().decode()
+    if code == SIE_ERR_INVALID:
+        raise SieInvalidArgument(code, msg)
+    if code == SIE_ERR_OVERFLOW:
+        raise SieOverflow(code, msg)
+    raise SieError(code, msg)
+
+
+class CalderonParams(ctypes.Structure):
+    _fields_ = [
+        ("rule", I),
+        ("k1", Fl), ("k2", Fl),
+        ("mu_r1", Fl), ("mu_r2", Fl), ("eps_r1", Fl), ("eps_r2", Fl),
+        ("mu0", Fl), ("eps0", Fl), ("omega", Fl),
+        ("threshold", Fl),
+    ]
+
+
+class PlaneWaveC(ctypes.Structure):
+    _fields_ = [("E0", D * 3), ("khat", D * 3), ("k", D), ("eta", D), ("region", I)]
+
+
+def _sig(name, res, args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = args
+    return f
+
+
+_sig("sie_last_error", ctypes.c_char_p, [])
+_sig("sie_mesh_create", I, [P, I, P, I, P, I, ctypes.POINTER(P)])
+_sig("sie_mesh_destroy", I, [P])
+_sig("sie_calderon_pair_blocks", I, [P, ctypes.POINTER(CalderonParams), P, P])
+_sig("sie_assemble_gstc", I, [P, ctypes.POINTER(CalderonParams), P, I, I, I, P, LL, P])
+_sig("sie_assemble_pec", I, [P, ctypes.POINTER(CalderonParams), P, LL, P])
+_sig("sie_assemble_pmchwt", I, [P, ctypes.POINTER(CalderonParams), P, LL, P])
+_sig("sie_rhs_plane_waves", I, [P, I, I, ctypes.POINTER(PlaneWaveC), I, Fl, Fl, D, D, P, LL, P])

@@ -158,3 +158,80 @@ def assemble_gstc(lib_path, prefix, mesh, rule, phys, tensors, uniform, basis, w
         fn(_ptr(np.ascontiguousarray(V)), _ptr(np.ascontiguousarray(F)), _ptr(np.ascontiguousarray(FE)),
            nf, ne, rule, *fl, _ptr(t), int(uniform), basis, int(with_mass), _ptr(Z))
     return Z
+
+
+def _fl(p, names):
+    return [F32(getattr(p, n)) for n in names]
+
+
+_PNAMES = ("k1", "k2", "mu_r1", "mu_r2", "eps_r1", "eps_r2", "mu0", "eps0", "omega", "threshold")
+
+
+def oracle_gstc(mesh, p, tensors, uniform, basis, with_calderon=True, with_mass=True):
+    """Oracle GSTC matrix from a CalderonParams struct (float fields)."""
+    lib = ctypes.CDLL(ORACLE_SO)
+    ne, nf = mesh.ne, mesh.nf
+    dim = 4 * ne
+    Z = np.zeros((dim, dim), complex, order="F")
+    fn = lib.oracle_assemble_gstc
+    fn.argtypes = [_P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int] + [ctypes.c_float] * 10 + \
+        [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P]
+    t = np.ascontiguousarray(tensors, complex)
+    fn(_ptr(mesh.vertices), _ptr(mesh.faces), _ptr(mesh.face_edges), nf, ne, p.rule, *_fl(p, _PNAMES),
+       _ptr(t), int(uniform), basis, int(with_calderon), int(with_mass), _ptr(Z))
+    return Z
+
+
+def oracle_pec(mesh, p):
+    lib = ctypes.CDLL(ORACLE_SO)
+    dim = mesh.ne
+    Z = np.zeros((dim, dim), complex, order="F")
+    fn = lib.oracle_assemble_pec
+    fn.argtypes = [_P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int] + [ctypes.c_float] * 5 + [_P]
+    fn(_ptr(mesh.vertices), _ptr(mesh.faces), _ptr(mesh.face_edges), mesh.nf, mesh.ne, p.rule,
+       F32(p.k2), F32(p.mu_r2), F32(p.mu0), F32(p.omega), F32(p.threshold), _ptr(Z))
+    return Z
+
+
+def oracle_pmchwt(mesh, p):
+    lib = ctypes.CDLL(ORACLE_SO)
+    dim = 2 * mesh.ne
+    Z = np.zeros((dim, dim), complex, order="F")
+    fn = lib.oracle_assemble_pmchwt
+    fn.argtypes = [_P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int] + [ctypes.c_float] * 10 + [_P]
+    fn(_ptr(mesh.vertices), _ptr(mesh.faces), _ptr(mesh.face_edges), mesh.nf, mesh.ne, p.rule,
+       *_fl(p, _PNAMES), _ptr(Z))
+    return Z
+
+
+def oracle_pair_blocks_all(mesh, p):
+    lib = ctypes.CDLL(ORACLE_SO)
+    out = np.zeros((mesh.nf, mesh.nf, 4, 3, 3), complex)
+    fn = lib.oracle_pair_blocks_all
+    fn.argtypes = [_P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P]
+    fn(_ptr(mesh.vertices), _ptr(mesh.faces), mesh.nf, p.rule, F32(p.k1), F32(p.k2), F32(p.threshold), _ptr(out))
+    return out
+
+
+def oracle_rhs(mesh, kind, rule, wave, mu0, eps0, vac_mu, vac_eps):
+    """kind 0 GSTC, 1 PMCHWT, 2 EFIE; wave: PlaneWaveC-like (E0, khat, k, eta, region)."""
+    lib = ctypes.CDLL(ORACLE_SO)
+    dim = {0: 4, 1: 2, 2: 1}[kind] * mesh.ne
+    rhs = np.zeros(dim, complex)
+    fn = lib.oracle_rhs_planewave
+    fn.argtypes = [_P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P, ctypes.c_double,
+                   ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                   ctypes.c_double, ctypes.c_double, _P]
+    E0 = np.array(list(wave.E0), float)
+    kh = np.array(list(wave.khat), float)
+    fn(_ptr(mesh.vertices), _ptr(mesh.faces), _ptr(mesh.face_edges), mesh.nf, mesh.ne, rule, _ptr(E0),
+       _ptr(kh), wave.k, wave.eta, wave.region, kind, F32(mu0), F32(eps0), vac_mu, vac_eps, _ptr(rhs))
+    return rhs
+
+
+def random_gstc_tensors(nf, seed=0):
+    """Random well-conditioned 4x4 maps and their inverses, 32 complex per face."""
+    rng = np.random.default_rng(seed)
+    lam = rng.normal(size=(nf, 4, 4)) + 1j * rng.normal(size=(nf, 4, 4)) + 4 * np.eye(4)
+    inv = np.linalg.inv(lam)
+    return np.concatenate([lam.reshape(nf, 16), inv.reshape(nf, 16)], axis=1).ravel()
