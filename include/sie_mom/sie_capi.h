@@ -126,6 +126,58 @@ int sie_rhs_plane_waves(sie_mesh_t mesh, int kind, int rule, const sie_plane_wav
                         int nwaves, float mu0, float eps0, double vac_mu, double vac_eps,
                         double* rhs_dev, long long ldb, void* stream);
 
+/* ------------------------------------------------------------ dense algebra */
+/* precision: 64 = complex double (interleaved doubles), 32 = complex float.
+ * op: 0 = N, 2 = C (conjugate transpose); the numbering follows
+ * cublasOperation_t (CUBLAS_OP_N = 0, CUBLAS_OP_C = 2), which the reference's
+ * DenseLuSolver::solve takes (dense_lu_solver.cuh:122-166). */
+
+/* C = beta C + alpha A B, complex column-major (m x k times k x n) on the
+ * matrix cores. variant (FP32 only): 0 = 4M real decomposition, 1 = 3M. */
+int sie_gemm(int precision, int m, int n, int k, double alpha, const void* A_dev, long long lda,
+             const void* B_dev, long long ldb, double beta, void* C_dev, long long ldc, int variant,
+             void* stream);
+
+/* In-place LU with partial pivoting, A = P L U (unit L), column-major n x n.
+ * ipiv_dev[k] (0-based) is the row swapped with row k. Replaces
+ * DenseLuSolver::factorize / cusolverDnZgetrf (dense_lu_solver.cuh:42-112);
+ * returns SIE_ERR_SINGULAR with "dense matrix is singular at U diagonal k"
+ * (k 1-based, also stored in *info) when a pivot column is exactly zero.
+ * Synchronises the stream. variant: as sie_gemm, for the trailing update. */
+int sie_lu_factor(int precision, int n, void* A_dev, long long lda, int* ipiv_dev, int* info,
+                  int variant, void* stream);
+
+/* Solve op(A) X = B in place with the factors of sie_lu_factor; B is n x nrhs
+ * (ldb). Replaces DenseLuSolver::solve / cusolverDnZgetrs
+ * (dense_lu_solver.cuh:122-166); op = 2 is the adjoint solve with A^H. */
+int sie_lu_solve(int precision, int op, int n, int nrhs, const void* LU_dev, long long lda,
+                 const int* ipiv_dev, void* B_dev, long long ldb, void* stream);
+
+/* y = beta y + alpha op(A) x, complex double, A m x n (lda). op 2 uses A^H
+ * (y has n entries). Replaces the cublasZgemv residual (gstc_sie.cu:398-457). */
+int sie_zgemv(int op, int m, int n, double alpha, const double* A_dev, long long lda,
+              const double* x_dev, double beta, double* y_dev, void* stream);
+
+/* B = scale * A with precision conversion (64->32, 32->64 or 64->64). */
+int sie_convert(int from_precision, int to_precision, long long rows, long long cols,
+                const void* A_dev, long long lda, void* B_dev, long long ldb, double scale,
+                void* stream);
+
+/* FP64 iterative refinement: solve op(Z) X = B with Z in complex double and
+ * an FP32 LU of Z (sie_lu_factor(32, ...)) as the inner solver:
+ *   x = 0, r = b;  repeat: x += |r| LU32^{-1}(r/|r|);  r = b - op(Z) x (FP64)
+ * until |r|/|b| <= tol or maxit sweeps. No counterpart in the reference (its
+ * solves are plain FP64 cuSOLVER); rel_residual[c] and iterations[c] report
+ * per column. Synchronises the stream every sweep (host norm). */
+int sie_ir_solve(int op, int n, int nrhs, const double* Z_dev, long long ldz, const float* LU_dev,
+                 long long ldl, const int* ipiv_dev, const double* B_dev, long long ldb, double* X_dev,
+                 long long ldx, double tol, int maxit, double* rel_residual, int* iterations,
+                 void* stream);
+
+/* Euclidean norm of n complex doubles (deterministic two-stage reduction,
+ * synchronises the stream; returns -1 on error). */
+double sie_norm2(long long n, const double* x_dev, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

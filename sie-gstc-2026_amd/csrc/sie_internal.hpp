@@ -50,7 +50,11 @@ struct DevBuf {
         n = count;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        // kernels still queued may reference the buffer: drain the device first
+        if (p) {
+            (void)hipDeviceSynchronize();
+            (void)hipFree(p);
+        }
         p = nullptr;
         n = 0;
     }

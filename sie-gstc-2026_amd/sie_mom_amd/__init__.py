@@ -6,4 +6,4 @@ from .physics import (  # noqa: F401
     calderon_params, plane_wave, frequency_for_k0,
 )
 from .mesh import Mesh  # noqa: F401
-from . import ops  # noqa: F401
+from . import ops, linalg  # noqa: F401

@@ -85,3 +85,10 @@ _sig("sie_assemble_gstc", I, [P, ctypes.POINTER(CalderonParams), P, I, I, I, P, 
 _sig("sie_assemble_pec", I, [P, ctypes.POINTER(CalderonParams), P, LL, P])
 _sig("sie_assemble_pmchwt", I, [P, ctypes.POINTER(CalderonParams), P, LL, P])
 _sig("sie_rhs_plane_waves", I, [P, I, I, ctypes.POINTER(PlaneWaveC), I, Fl, Fl, D, D, P, LL, P])
+_sig("sie_gemm", I, [I, I, I, I, D, P, LL, P, LL, D, P, LL, I, P])
+_sig("sie_lu_factor", I, [I, I, P, LL, P, ctypes.POINTER(I), I, P])
+_sig("sie_lu_solve", I, [I, I, I, I, P, LL, P, P, LL, P])
+_sig("sie_zgemv", I, [I, I, I, D, P, LL, P, D, P, P])
+_sig("sie_convert", I, [I, I, LL, LL, P, LL, P, LL, D, P])
+_sig("sie_ir_solve", I, [I, I, I, P, LL, P, LL, P, P, LL, P, LL, D, I, P, P, P])
+_sig("sie_norm2", D, [LL, P, P])

@@ -1,0 +1,115 @@
+"""Dense complex linear algebra on the GPU: LU (FP64 or FP32 matrix cores),
+triangular solves with op N / C, GEMV and FP64 iterative refinement.
+
+Mirrors the reference's operator-level DenseLuSolver
+(include/sie_mom/cuda/dense_lu_solver.cuh:17-197): factorize(A) then
+solve(op, nrhs, B); op "N" solves A X = B, op "C" solves A^H X = B.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .ops import _dptr, _stream, colmajor
+
+OPS = {"N": 0, "C": 2}
+
+
+def _prec(t):
+    if t.dtype == torch.complex128:
+        return 64
+    if t.dtype == torch.complex64:
+        return 32
+    raise TypeError("complex64 or complex128 device tensors expected")
+
+
+def _ld(A):
+    if A.stride(0) != 1:
+        raise ValueError("column-major storage (stride(0) == 1) required")
+    return A.stride(1) if A.dim() == 2 and A.shape[1] > 1 else max(A.shape[0], 1)
+
+
+def gemm(A, B, C=None, alpha=1.0, beta=0.0, variant="4m"):
+    """C = beta C + alpha A B on the matrix cores (column-major operands)."""
+    m, k = A.shape
+    k2, n = B.shape
+    assert k == k2
+    if C is None:
+        C = colmajor(m, n, dtype=A.dtype)
+        beta = 0.0
+    _lib.check(_lib.lib.sie_gemm(_prec(A), m, n, k, float(alpha), _dptr(A), _ld(A), _dptr(B), _ld(B),
+                                 float(beta), _dptr(C), _ld(C), 1 if variant == "3m" else 0, _stream()))
+    return C
+
+
+class DenseLuSolver:
+    """In-place LU with partial pivoting of a column-major square matrix."""
+
+    def __init__(self, dimension=0, variant="4m"):
+        if dimension < 0:
+            raise ValueError("DenseLuSolver dimension must be positive")
+        self.dimension = dimension
+        self.variant = variant
+        self.pivots = None
+        self.factors = None
+
+    def factorize(self, A):
+        n = A.shape[0]
+        if A.shape[1] != n:
+            raise ValueError("square matrix expected")
+        self.dimension = n
+        self.pivots = torch.empty(n, dtype=torch.int32, device=A.device)
+        info = ctypes.c_int(0)
+        _lib.check(_lib.lib.sie_lu_factor(_prec(A), n, _dptr(A), _ld(A), _dptr(self.pivots), ctypes.byref(info),
+                                          1 if self.variant == "3m" else 0, _stream()))
+        self.factors = A
+        return A
+
+    def solve(self, B, op="N", factored=None):
+        """Solve op(A) X = B in place (B: n or n x nrhs column-major)."""
+        LU = self.factors if factored is None else factored
+        n = self.dimension
+        Bm = B if B.dim() == 2 else B.view(n, 1)
+        nrhs = Bm.shape[1]
+        ldb = Bm.stride(1) if nrhs > 1 else n
+        if Bm.stride(0) != 1:
+            raise ValueError("column-major right-hand side required")
+        _lib.check(_lib.lib.sie_lu_solve(_prec(LU), OPS[op], n, nrhs, _dptr(LU), _ld(LU), _dptr(self.pivots),
+                                         _dptr(Bm), ldb, _stream()))
+        return B
+
+
+def zgemv(A, x, y=None, alpha=1.0, beta=0.0, op="N"):
+    m, n = A.shape
+    out = (n if op == "C" else m)
+    if y is None:
+        y = torch.zeros(out, dtype=torch.complex128, device=A.device)
+    _lib.check(_lib.lib.sie_zgemv(OPS[op], m, n, float(alpha), _dptr(A), _ld(A), _dptr(x), float(beta),
+                                  _dptr(y), _stream()))
+    return y
+
+
+def to_precision(A, precision):
+    """Column-major copy of A in complex64 (32) or complex128 (64)."""
+    rows, cols = A.shape
+    dtype = torch.complex64 if precision == 32 else torch.complex128
+    B = colmajor(rows, cols, dtype=dtype)
+    _lib.check(_lib.lib.sie_convert(_prec(A), precision, rows, cols, _dptr(A), _ld(A), _dptr(B), _ld(B), 1.0,
+                                    _stream()))
+    return B
+
+
+def ir_solve(Z, LU32, pivots, B, op="N", tol=1e-12, maxit=30):
+    """FP64 iterative refinement with an FP32 LU. Returns (X, rel_residuals, iterations)."""
+    n = Z.shape[0]
+    Bm = B if B.dim() == 2 else B.view(n, 1)
+    nrhs = Bm.shape[1]
+    X = colmajor(n, nrhs)
+    rr = np.zeros(nrhs)
+    its = np.zeros(nrhs, np.int32)
+    _lib.check(_lib.lib.sie_ir_solve(OPS[op], n, nrhs, _dptr(Z), _ld(Z), _dptr(LU32), _ld(LU32), _dptr(pivots),
+                                     _dptr(Bm), Bm.stride(1) if nrhs > 1 else n, _dptr(X), X.stride(1) if nrhs > 1 else n,
+                                     float(tol), int(maxit), rr.ctypes.data_as(ctypes.c_void_p),
+                                     its.ctypes.data_as(ctypes.c_void_p), _stream()))
+    return (X if B.dim() == 2 else X[:, 0]), rr, its

@@ -1,0 +1,115 @@
+"""GPU dense algebra against numpy/scipy (textbook LAPACK routines)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def sm():
+    import sie_mom_amd
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return sie_mom_amd
+
+
+def crandn(rng, *shape):
+    return rng.normal(size=shape) + 1j * rng.normal(size=shape)
+
+
+def to_dev(a, dtype=torch.complex128):
+    """numpy (column-major semantics) -> column-major device tensor."""
+    t = torch.from_numpy(np.ascontiguousarray(a.T)).to("cuda", dtype)
+    return t.T
+
+
+@pytest.mark.parametrize("prec,variant", [(64, "4m"), (32, "4m"), (32, "3m")])
+@pytest.mark.parametrize("m,n,k", [(128, 128, 16), (300, 200, 70), (513, 65, 256), (1, 1, 1), (1000, 3, 5)])
+def test_gemm(sm, prec, variant, m, n, k):
+    rng = np.random.default_rng(m + n + k)
+    A, B, C = crandn(rng, m, k), crandn(rng, k, n), crandn(rng, m, n)
+    dt = torch.complex128 if prec == 64 else torch.complex64
+    Cd = to_dev(C, dt)
+    print(f"gemm prec={prec} {variant} m={m} n={n} k={k}", flush=True)
+    sm.linalg.gemm(to_dev(A, dt), to_dev(B, dt), Cd, alpha=-1.0, beta=1.0, variant=variant)
+    ref = C - A @ B
+    err = np.abs(Cd.cpu().numpy() - ref).max() / (np.abs(A).max() * np.abs(B).max() * k)
+    assert err < (1e-15 if prec == 64 else 2e-7), err
+
+
+def well_conditioned(rng, n):
+    return crandn(rng, n, n) + 2 * np.sqrt(n) * np.eye(n)
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("n", [16, 100, 300, 513, 1024])
+def test_lu_solve(sm, prec, n):
+    rng = np.random.default_rng(n)
+    A = crandn(rng, n, n)  # general (pivoting exercised)
+    b = crandn(rng, n, 3)
+    dt = torch.complex128 if prec == 64 else torch.complex64
+    Ad = to_dev(A, dt)
+    print(f"lu prec={prec} n={n}", flush=True)
+    lu = sm.linalg.DenseLuSolver()
+    lu.factorize(Ad)
+    for op in ("N", "C"):
+        Bd = to_dev(b, dt)
+        lu.solve(Bd, op=op)
+        X = Bd.cpu().numpy()
+        M = A if op == "N" else A.conj().T
+        res = np.linalg.norm(M @ X - b) / (np.linalg.norm(M) * np.linalg.norm(X))
+        assert res < (1e-14 if prec == 64 else 5e-6), (op, res)
+    # factors reproduce P A = L U
+    F = Ad.cpu().numpy()
+    L = np.tril(F, -1) + np.eye(n)
+    U = np.triu(F)
+    piv = lu.pivots.cpu().numpy()
+    PA = A.copy()
+    for k in range(n):
+        p = piv[k]
+        PA[[k, p]] = PA[[p, k]]
+    err = np.linalg.norm(PA - L @ U) / np.linalg.norm(A)
+    assert err < (1e-14 if prec == 64 else 1e-5), err
+    assert np.all(np.abs(L) <= 1 + 1e-6)  # partial pivoting bound
+
+
+def test_singular_reported(sm):
+    n = 64
+    rng = np.random.default_rng(1)
+    A = crandn(rng, n, n)
+    A[:, 10] = 0.0
+    lu = sm.linalg.DenseLuSolver()
+    with pytest.raises(sm.SieError, match="singular at U diagonal 11"):
+        lu.factorize(to_dev(A))
+
+
+@pytest.mark.parametrize("op", ["N", "C"])
+def test_zgemv(sm, op):
+    rng = np.random.default_rng(5)
+    m, n = 700, 3000
+    A = crandn(rng, m, n)
+    x = crandn(rng, n if op == "N" else m)
+    y0 = crandn(rng, m if op == "N" else n)
+    y = torch.from_numpy(y0).cuda()
+    sm.linalg.zgemv(to_dev(A), torch.from_numpy(x).cuda(), y, alpha=-1.0, beta=1.0, op=op)
+    M = A if op == "N" else A.conj().T
+    ref = y0 - M @ x
+    assert np.abs(y.cpu().numpy() - ref).max() / np.abs(ref).max() < 1e-13
+
+
+@pytest.mark.parametrize("op", ["N", "C"])
+def test_ir_solve(sm, op):
+    rng = np.random.default_rng(9)
+    n = 700
+    A = crandn(rng, n, n) + 5 * np.sqrt(n) * np.eye(n)
+    b = crandn(rng, n, 2)
+    Zd = to_dev(A)
+    LU32 = sm.linalg.to_precision(Zd, 32)
+    lu = sm.linalg.DenseLuSolver()
+    lu.factorize(LU32)
+    X, rr, its = sm.linalg.ir_solve(Zd, LU32, lu.pivots, to_dev(b), op=op, tol=1e-13, maxit=20)
+    M = A if op == "N" else A.conj().T
+    xr = np.linalg.solve(M, b)
+    assert np.all(rr < 1e-13)
+    assert np.linalg.norm(X.cpu().numpy() - xr) / np.linalg.norm(xr) < 1e-12
