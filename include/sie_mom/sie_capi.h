@@ -178,6 +178,48 @@ int sie_ir_solve(int op, int n, int nrhs, const double* Z_dev, long long ldz, co
  * synchronises the stream; returns -1 on error). */
 double sie_norm2(long long n, const double* x_dev, void* stream);
 
+/* ----------------------------------------------------------- adjoint */
+/* GSTC surface maps from real per-face parameters (kind = parameters per
+ * face): Lambda = [[jB, K], [-K^T, jX]] (the reciprocal lossless form of
+ * src/sie_mom/gstc/surface_parameters.py:280-306) with
+ *   kind 3  (omega, omega_biisotropic.py:196-219): B = b p0 I, X = x p1 I,
+ *            K = k p2 [[0, 1], [-1, 0]];
+ *   kind 10 (general 2x2): B = b [[p0, p2], [p2, p1]], X = x [[p3, p5], [p5, p4]],
+ *            K = k [[p6, p7], [p8, p9]],
+ * where b, x, k are b_scale, x_scale, k_scale (e.g. w eps0, w mu0, k0 for
+ * susceptibility parameters). Writes the 32-complex records per face
+ * (Lambda row-major, then Lambda^-1 inverted in impedance-balanced form with
+ * eta_ref, as src/sie_mom/gstc/tensor_io.py:17-130 does); *n_singular counts
+ * faces whose map could not be inverted (then SIE_ERR_SINGULAR). */
+int sie_gstc_records(int nf, int kind, const double* params_dev, double b_scale, double x_scale,
+                     double k_scale, double eta_ref, double* records_dev, int* n_singular,
+                     void* stream);
+
+/* Far field of the exterior currents of a scaled GSTC solution xs (dim 4E,
+ * physical J2 = xs[2E:3E], M2 = eta0 xs[3E:4E]) in ndir directions. dirs is a
+ * host array of ndir x 9 doubles (r_hat, theta_hat, phi_hat); F_dev receives
+ * ndir x 2 complex (F_theta, F_phi) with the Schelkunoff convention of
+ * applications/anisotropic_gstc_sphere/plot_far_field.py:150-219. */
+int sie_farfield(sie_mesh_t mesh, int rule, double k0, double eta0, const double* x_dev, int ndir,
+                 const double* dirs, double* F_dev, void* stream);
+
+/* Adjoint of sie_farfield: g = A^H u for F = A xs, with u_dev ndir x 2
+ * complex. g_dev (dim 4E complex) is the gradient dJ/d conj(xs) of
+ * J = sum_d (u_d . conj F_d) when u_d = w_d F_d / (2 eta0). */
+int sie_farfield_adjoint(sie_mesh_t mesh, int rule, double k0, double eta0, int ndir, const double* dirs,
+                         const double* u_dev, double* g_dev, void* stream);
+
+/* Per-face derivative of S = lambda^H Zs_mass x with respect to the 32 record
+ * entries (Lambda then Lambda^-1) of that face: G_dev nf x 32 complex. Zs_mass
+ * is the eta0-scaled GSTC mass operator of sie_assemble_gstc. */
+int sie_gstc_mass_sensitivity(sie_mesh_t mesh, int rule, int basis, float mu0, float eps0,
+                              const double* lambda_dev, const double* x_dev, double* G_dev, void* stream);
+
+/* Chain rule to the real parameters: grad_dev (nf x kind) = -2 Re dS/dtheta
+ * with dLambda^-1 = -Lambda^-1 dLambda Lambda^-1. */
+int sie_gstc_param_grad(int nf, int kind, const double* records_dev, const double* G_dev,
+                        double b_scale, double x_scale, double k_scale, double* grad_dev, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

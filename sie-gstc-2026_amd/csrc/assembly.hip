@@ -898,6 +898,13 @@ struct sie_mesh {
 
 namespace sie {
 
+Mesh& mesh_of(sie_mesh* h) { return h->m; }
+const double* mesh_area(sie_mesh* h) { return h->area.p; }
+QuadDev& mesh_quad(sie_mesh* h, int rule, hipStream_t s) {
+    quad_upload(h->as->quad, rule, s);
+    return h->as->quad;
+}
+
 static int NKof(const sie_calderon_params* p) { return (p->k1 == p->k2) ? 1 : 2; }
 
 template <int NK, int KIND>

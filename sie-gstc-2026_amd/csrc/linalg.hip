@@ -7,7 +7,8 @@
 //  * cgemm_mfma<T>: C = beta C + alpha A B for complex column-major blocks on
 //    the matrix cores. FP32: v_mfma_f32_32x32x2f32 with a 128x128 workgroup
 //    tile, real 4M (four real products) or 3M (Gauss) decomposition; FP64:
-//    v_mfma_f64_16x16x4f64 with a 64x128 tile, 4M. The roles of A and B are
+//    v_mfma_f64_16x16x4f64 with a 64x128 tile, 4M (its D layout: lane l,
+//    register v holds D[4v + l/16][l%16]). The roles of A and B are
 //    swapped in the MFMA so that each lane owns rows of C: the epilogue's
 //    column-major stores are contiguous across lanes.
 //  * panel_getf2<T>: the partial-pivoting factorisation of a tall panel of
@@ -237,7 +238,8 @@ __global__ __launch_bounds__(256, 2) void zgemm_f64_kernel(int M, int N, int K, 
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const long long i = i0 + wm * 32 + ti * 16 + (lane & 15);
-                const long long j = j0 + wn * 64 + tj * 16 + 4 * (lane >> 4) + v;
+                // v_mfma_f64_16x16x4f64: lane l, register v holds D[4v + l/16][l%16]
+                const long long j = j0 + wn * 64 + tj * 16 + 4 * v + (lane >> 4);
                 if (i < M && j < N) {
                     c64* p = C + j * ldc + i;
                     c64 c = beta != 0.0 ? *p : c64{0.0, 0.0};

@@ -235,3 +235,59 @@ def random_gstc_tensors(nf, seed=0):
     lam = rng.normal(size=(nf, 4, 4)) + 1j * rng.normal(size=(nf, 4, 4)) + 4 * np.eye(4)
     inv = np.linalg.inv(lam)
     return np.concatenate([lam.reshape(nf, 16), inv.reshape(nf, 16)], axis=1).ravel()
+
+
+# ------------------------------------------------------------------ adjoint
+def oracle_records(params, kind, bs, xs, ks, eta):
+    """Lambda = [[jB, K], [-K^T, jX]] per face and its inverse (numpy), the
+    record layout of src/sie_mom/gstc/tensor_io.py:118-120."""
+    nf = len(params)
+    L = np.zeros((nf, 4, 4), complex)
+    p = np.asarray(params, float)
+    if kind == 3:
+        L[:, 0, 0] = L[:, 1, 1] = 1j * bs * p[:, 0]
+        L[:, 2, 2] = L[:, 3, 3] = 1j * xs * p[:, 1]
+        k = ks * p[:, 2]
+        L[:, 0, 3], L[:, 1, 2], L[:, 2, 1], L[:, 3, 0] = k, -k, k, -k
+    else:
+        L[:, 0, 0], L[:, 1, 1] = 1j * bs * p[:, 0], 1j * bs * p[:, 1]
+        L[:, 0, 1] = L[:, 1, 0] = 1j * bs * p[:, 2]
+        L[:, 2, 2], L[:, 3, 3] = 1j * xs * p[:, 3], 1j * xs * p[:, 4]
+        L[:, 2, 3] = L[:, 3, 2] = 1j * xs * p[:, 5]
+        K = ks * p[:, 6:10].reshape(nf, 2, 2)
+        L[:, 0:2, 2:4] = K
+        L[:, 2:4, 0:2] = -np.transpose(K, (0, 2, 1))
+    t = np.array([np.sqrt(eta)] * 2 + [1 / np.sqrt(eta)] * 2)
+    bal = L * t[None, :, None] * t[None, None, :]
+    inv = np.linalg.inv(bal) * t[None, :, None] * t[None, None, :]
+    return np.concatenate([L.reshape(nf, 16), inv.reshape(nf, 16)], axis=1).ravel()
+
+
+def oracle_farfield_matrix(mesh, rule, k0, eta0, dirs):
+    """A (ndir*2 x 4E): [F_theta, F_phi] = A xs for scaled currents xs
+    (Schelkunoff, plot_far_field.py:150-219)."""
+    xy, w = oracle().rule(rule)
+    xy = xy.astype(float)
+    w = w.astype(float)
+    V, F, FE = mesh.vertices, mesh.faces, mesh.face_edges
+    ne = mesh.ne
+    nd = len(dirs)
+    A = np.zeros((nd, 2, 4 * ne), complex)
+    coef = -1j * k0 / (4 * np.pi)
+    for f in range(len(F)):
+        v = V[F[f]]
+        r = (v[0][None, :] * (1 - xy[:, 0:1] - xy[:, 1:2]) + v[1][None, :] * xy[:, 0:1] + v[2][None, :] * xy[:, 1:2])
+        ph = np.exp(1j * k0 * (dirs[:, 0:3] @ r.T)) * w[None, :]  # nd x Q
+        for le in range(3):
+            a, b = F[f, le], F[f, (le + 1) % 3]
+            length = np.linalg.norm(v[(le + 1) % 3] - v[le]) * (1.0 if a > b else -1.0)
+            rho = r - v[(le + 2) % 3][None, :]          # Q x 3
+            n = 0.5 * length * (ph @ rho)                 # nd x 3
+            nt = np.einsum("dk,dk->d", n, dirs[:, 3:6])
+            npp = np.einsum("dk,dk->d", n, dirs[:, 6:9])
+            e = FE[f, le]
+            A[:, 0, 2 * ne + e] += coef * eta0 * nt
+            A[:, 1, 2 * ne + e] += coef * eta0 * npp
+            A[:, 0, 3 * ne + e] += coef * eta0 * npp
+            A[:, 1, 3 * ne + e] += -coef * eta0 * nt
+    return A.reshape(nd * 2, 4 * ne)

@@ -70,8 +70,9 @@ def test_lu_solve(sm, prec, n):
         p = piv[k]
         PA[[k, p]] = PA[[p, k]]
     err = np.linalg.norm(PA - L @ U) / np.linalg.norm(A)
-    assert err < (1e-14 if prec == 64 else 1e-5), err
-    assert np.all(np.abs(L) <= 1 + 1e-6)  # partial pivoting bound
+    assert err < (4 * n * 1.2e-16 if prec == 64 else 4 * n * 6e-8), err
+    # pivots maximise |re| + |im| (LAPACK izamax), so |l_ij| <= sqrt(2)
+    assert np.all(np.abs(L) <= np.sqrt(2) * (1 + 1e-6))
 
 
 def test_singular_reported(sm):
