@@ -1,0 +1,270 @@
+#!/usr/bin/env python3
+"""SIE-GSTC synthesis iteration benchmark on one MI355X.
+
+Workload (default): the anisotropic GSTC metasurface on a circular-arc
+spindle (arc radius 1.5 lambda about its 2 lambda chord), N ~ 72k unknowns,
+10 real susceptibility parameters per face drawn uniformly from [-2, 2]/k0
+(seeded), one x-polarised plane wave along +z, Dunavant rule 17 (the GSTC
+default), objective = far-field power in the 10-degree cone about +z.
+
+One step = one synthesis iteration: surface maps from the parameters ->
+FP64 eta0-scaled assembly -> complex-FP32 LU on the matrix cores -> FP64
+iterative refinement -> objective -> adjoint solve with Z^H (the same FP32
+factors + FP64 refinement) -> per-face gradient contraction -> parameter
+update. The plain FP64 path (FP64 LU, FP64 solves) runs once afterwards for
+the speed ratio and the error of the mixed path against it.
+
+Prints one JSON line (rank 0). value = seconds per iteration (lower is better).
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "sie-gstc-2026_amd"))
+
+METRIC = "assemble+solve+adjoint-grad s/iter @N≈72k, FP32-MFMA+IR vs FP64, err≤1e-8"
+
+WORKLOADS = {
+    # name: (mesh kind, size parameter, quadrature rule, parameters per face)
+    "gstc-spindle-72k": ("spindle", 91, 17, 10),
+    "gstc-spindle-31k": ("spindle", 60, 17, 10),
+    "gstc-sphere-30k": ("sphere", 4, 17, 3),
+    "gstc-spindle-72k-rule7": ("spindle", 91, 7, 10),
+    "gstc-spindle-8k": ("spindle", 30, 7, 10),
+}
+
+PROF = {"gemm_f32": 0, "gemm_f64": 1, "far_pairs": 2, "panel": 3}
+PEAK = {"gemm_f32": (157.3e12, "mfma"), "gemm_f64": (78.6e12, "mfma"), "far_pairs": (78.6e12, "valu-fp64"),
+        "panel": (8.0e12, "hbm")}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def prof_read(lib, k):
+    n = ctypes.c_int()
+    ms, fl, by = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    lib.sie_profile_read(k, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(fl), ctypes.byref(by))
+    return n.value, ms.value, fl.value, by.value
+
+
+def cpu_baseline(mesh, o, kind, seconds):
+    """Reference host numerics (oracle/_ref) on a bounded sample of the same
+    workload, extrapolated to one iteration: the Calderon assembly of a subset
+    of test faces against all source faces (rule as configured) and a dense
+    complex128 LU (OpenBLAS zgetrf through scipy) at a smaller dimension,
+    scaled by F^2 and N^3. The adjoint and refinement sweeps are O(N^2) and
+    are added as two triangular-solve pairs at measured zgetrs speed."""
+    import numpy as np
+    import scipy.linalg
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    if not ol.ref_available():
+        return None
+    lib = ctypes.CDLL(ol.REF_SO)
+    f = lib.ref_sample_pairs
+    P = ctypes.c_void_p
+    f.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                  ctypes.c_float]
+    f.restype = ctypes.c_double
+    import sie_mom_amd as sm
+    p = sm.calderon_params(o)
+    V = mesh.vertices
+    F = mesh.faces
+    nf = mesh.nf
+    # calibrate rows so that the sample takes ~seconds/2
+    rows = 2
+    t0 = time.perf_counter()
+    f(V.ctypes.data_as(P), F.ctypes.data_as(P), nf, 0, rows, p.rule, p.k1, p.k2, p.threshold)
+    dt = time.perf_counter() - t0
+    rows2 = max(2, min(nf, int(rows * (0.5 * seconds) / max(dt, 1e-3))))
+    t0 = time.perf_counter()
+    f(V.ctypes.data_as(P), F.ctypes.data_as(P), nf, 0, rows2, p.rule, p.k1, p.k2, p.threshold)
+    dt = time.perf_counter() - t0
+    asm = dt * nf / rows2
+    # LU at a smaller n
+    n_s = 3072
+    rng = np.random.default_rng(0)
+    A = rng.normal(size=(n_s, n_s)) + 1j * rng.normal(size=(n_s, n_s))
+    t0 = time.perf_counter()
+    lu = scipy.linalg.lu_factor(A, overwrite_a=True, check_finite=False)
+    tlu = time.perf_counter() - t0
+    b = rng.normal(size=n_s) + 0j
+    t0 = time.perf_counter()
+    for _ in range(4):
+        scipy.linalg.lu_solve(lu, b, check_finite=False)
+    tsolve = (time.perf_counter() - t0) / 4
+    N = 4 * mesh.ne
+    lu_ext = tlu * (N / n_s) ** 3
+    solve_ext = tsolve * (N / n_s) ** 2
+    total = asm + lu_ext + 2 * solve_ext
+    threads = os.cpu_count()
+    try:
+        threads = len(os.sched_getaffinity(0))
+    except Exception:
+        pass
+    return {"value": total, "unit": "s/iter", "cores": threads, "kind": "reference",
+            "sample": (f"reference host Calderon assembly of {rows2} of {nf} test faces x {nf} source faces "
+                       f"({dt:.1f} s, x{nf / rows2:.1f}) + scipy/OpenBLAS zgetrf n={n_s} ({tlu:.2f} s, x(N/n)^3) "
+                       f"+ 2 zgetrs sweeps; extrapolated to N={N}"),
+            "assembly_s": asm, "lu_s": lu_ext}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--workload", default="gstc-spindle-72k", choices=sorted(WORKLOADS))
+    ap.add_argument("--variant", default="4m", choices=["4m", "3m"])
+    ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison path")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the dense SIE-MoM path is "
+              "single-GPU here and its multi-GPU sharding is a design note, not a measured path.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import sie_mom_amd as sm
+    from sie_mom_amd import _lib
+    from sie_mom_amd import adjoint as ad
+
+    kind_mesh, size, rule, kind = WORKLOADS[args.workload]
+    lam = 1.0
+    k0 = 2 * math.pi / lam
+    if kind_mesh == "spindle":
+        mesh = sm.Mesh.spindle(2 * lam, 1.5 * lam, size)
+    else:
+        mesh = sm.Mesh.icosphere(1.0, size)
+        k0 = 4.0
+    o = sm.GstcOptions(frequency_hz=sm.frequency_for_k0(k0), quadrature_rule=rule)
+    waves = [sm.plane_wave(o, (1, 0, 0), (0, 0, 1), "exterior")]
+    N = 4 * mesh.ne
+    log(f"workload {args.workload}: F={mesh.nf} E={mesh.ne} N={N} rule={rule} params/face={kind}")
+    rng = np.random.default_rng(20260101)
+    params = torch.from_numpy(rng.uniform(-2, 2, size=(mesh.nf, kind)) / k0).cuda()
+    problem = ad.GstcProblem(mesh, o, waves, kind, mode="mixed", ir_tol=1e-12, ir_maxit=40, variant=args.variant)
+    lib = _lib.lib
+    # load every kernel once on a tiny problem (code objects, workspaces)
+    tiny = sm.Mesh.icosphere(1.0, 1)
+    tp = ad.GstcProblem(tiny, sm.GstcOptions(frequency_hz=o.frequency_hz, quadrature_rule=rule), waves, kind)
+    tst = tp.solve(torch.zeros((tiny.nf, kind), dtype=torch.float64) + 0.1 / k0)
+    tJ, tF = tp.objective(tst)
+    tp.gradient(tst, tF)
+    del tst, tp
+    torch.cuda.synchronize()
+
+    def step(p):
+        st = problem.solve(p)
+        J, F = problem.objective(st)
+        g = problem.gradient(st, F)
+        # gradient ascent on the cone power, parameter step bounded by 1e-3/k0
+        p = p + (1e-3 / k0) * g / g.abs().max()
+        return p, st, J, g
+
+    for _ in range(args.warmup):
+        params, st, J, g = step(params)
+        del st
+    for k in PROF.values():
+        lib.sie_profile_enable(k, 1)
+    problem.timings = {}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    p_start = params.clone()
+    for _ in range(args.steps):
+        params, st, J, g = step(params)
+        if _ < args.steps - 1:
+            del st
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    s_per_iter = elapsed / args.steps
+    ir_res, ir_its = st.ir_residual, st.ir_iterations
+    prof = {name: prof_read(lib, k) for name, k in PROF.items()}
+    for k in PROF.values():
+        lib.sie_profile_enable(k, 0)
+    phases = {k: v / args.steps for k, v in problem.timings.items()}
+    log(f"mixed: {s_per_iter:.3f} s/iter  phases {json.dumps({k: round(v, 3) for k, v in phases.items()})}")
+    log(f"IR: rel residual {ir_res:.2e} in {ir_its} sweeps; J={float(J):.6e}")
+
+    # re-run the last iterate's solve in FP64 for the error and the speed ratio
+    out_extra = {}
+    if not args.no_fp64:
+        p_last = params - (1e-3 / k0) * g / g.abs().max()  # the parameters of the last timed solve
+        x_mixed = st.xs.clone()
+        g_mixed = g.clone()
+        J_mixed = float(J)
+        del st
+        torch.cuda.empty_cache()
+        p64 = ad.GstcProblem(mesh, o, waves, kind, mode="fp64")
+        lib.sie_profile_enable(PROF["gemm_f64"], 1)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        st64 = p64.solve(p_last)
+        J64, F64 = p64.objective(st64)
+        g64 = p64.gradient(st64, F64)
+        torch.cuda.synchronize()
+        t64 = time.perf_counter() - t1
+        prof["gemm_f64"] = prof_read(lib, PROF["gemm_f64"])
+        lib.sie_profile_enable(PROF["gemm_f64"], 0)
+        ex = float(torch.linalg.norm(x_mixed - st64.xs) / torch.linalg.norm(st64.xs))
+        eg = float(torch.linalg.norm(g_mixed - g64) / torch.linalg.norm(g64))
+        eJ = abs(J_mixed - float(J64)) / abs(float(J64))
+        log(f"fp64: {t64:.3f} s/iter  phases {json.dumps({k: round(v, 3) for k, v in p64.timings.items()})}")
+        log(f"mixed vs fp64: rel err x {ex:.2e}, grad {eg:.2e}, J {eJ:.2e}")
+        out_extra = {"fp64_s_per_iter": t64, "speedup_vs_fp64": t64 / s_per_iter, "rel_err_x_vs_fp64": ex,
+                     "rel_err_grad_vs_fp64": eg, "rel_err_J_vs_fp64": eJ,
+                     "fp64_phases_s": {k: round(v, 4) for k, v in p64.timings.items()}}
+        del st64
+        torch.cuda.empty_cache()
+
+    rooflines = {}
+    for name, (n, ms, fl, by) in prof.items():
+        if n == 0 or ms <= 0:
+            continue
+        peak, bound = PEAK[name]
+        if bound == "hbm":
+            ach = by / (ms / n * 1e-3) / n
+            rooflines[name] = {"bound": "hbm", "achieved": ach / 1e9, "peak": peak / 1e9, "unit": "GB/s",
+                               "frac": ach / peak, "launches": n, "avg_ms": ms / n}
+        else:
+            ach = fl / (ms * 1e-3)
+            rooflines[name] = {"bound": bound, "achieved": ach / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+                               "frac": ach / peak, "launches": n, "avg_ms": ms / n, "total_ms": ms}
+    dom = max(("gemm_f32", "far_pairs"), key=lambda k: prof[k][1])
+    r = dict(rooflines.get(dom, {}))
+    roof = {"kernel": dom, "bound": "mfma" if dom == "gemm_f32" else "valu-fp64",
+            "achieved": r.get("achieved"), "peak": r.get("peak"), "unit": "TFLOP/s", "frac": r.get("frac"),
+            "traffic": None, "avg_ms": r.get("avg_ms"), "launches": r.get("launches")}
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(mesh, o, kind, args.cpu_seconds)
+        except Exception as e:  # the baseline is reported, never required
+            log(f"cpu baseline skipped: {e}")
+    line = {
+        "metric": METRIC, "value": s_per_iter, "unit": "s/iter", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": s_per_iter * 1e3, "higher_is_better": False, "scaling": "strong",
+        "vs_baseline": None, "dtype": "fp32-mfma-lu+fp64", "data": "synthetic (seeded parameters, generated mesh)",
+        "config": {"workload": args.workload, "faces": mesh.nf, "edges": mesh.ne, "unknowns": N,
+                   "quadrature_rule": rule, "params_per_face": kind, "rhs": 1, "lu_variant": args.variant,
+                   "objective": "far-field power, 10 deg cone about +z", "ir_tol": 1e-12},
+        "phases_s": {k: round(v, 4) for k, v in phases.items()},
+        "ir_rel_residual": ir_res, "ir_sweeps": ir_its,
+        "roofline": roof, "rooflines": rooflines, "cpu_baseline": cpu,
+    }
+    line.update(out_extra)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -220,6 +220,15 @@ int sie_gstc_mass_sensitivity(sie_mesh_t mesh, int rule, int basis, float mu0, f
 int sie_gstc_param_grad(int nf, int kind, const double* records_dev, const double* G_dev,
                         double b_scale, double x_scale, double k_scale, double* grad_dev, void* stream);
 
+/* ------------------------------------------------------------ profiling */
+/* Per-kernel timing with HIP events on the launch stream (no counterpart in
+ * the reference, which times phases with std::chrono). kernel: 0 FP32 GEMM,
+ * 1 FP64 GEMM, 2 far-pair assembly, 3 LU panel. enable(k, 1) clears and
+ * starts recording; read() synchronises on the recorded events and returns
+ * the launch count, summed duration and summed algorithmic flops / bytes. */
+int sie_profile_enable(int kernel, int on);
+int sie_profile_read(int kernel, int* launches, double* total_ms, double* flops, double* bytes);
+
 #ifdef __cplusplus
 }
 #endif

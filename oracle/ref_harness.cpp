@@ -386,3 +386,26 @@ extern "C" double ref_eta0_calderon(float mu0, float eps0) {
     const real_t eta0 = sqrt(mu0 / eps0);
     return eta0;
 }
+
+// Reference host numerics on a sample of the Calderon assembly: test faces
+// [m0, m1) against every source face, with the kernel's near/far branching
+// (sie_kernels_assembly.cuh:187-220). OpenMP over test faces. Returns a
+// checksum so the work cannot be elided.
+extern "C" double ref_sample_pairs(const double* V, const int* F, int nf, int m0, int m1, int rule, float k1,
+                                   float k2, float thr) {
+    Quad q = quad(rule);
+    double sum = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : sum)
+    for (int m = m0; m < m1; ++m) {
+        vec3_t tt[3]; int ti[3];
+        load_face(V, F, m, tt, ti);
+        for (int n = 0; n < nf; ++n) {
+            vec3_t st[3]; int si[3];
+            load_face(V, F, n, st, si);
+            complex__float tl1[3][3], tl2[3][3], tk1[3][3], tk2[3][3];
+            pair_blocks(tt, ti, st, si, q, k1, k2, thr, tl1, tl2, tk1, tk2);
+            sum += tl1[0][0].x + tk2[2][2].y;
+        }
+    }
+    return sum;
+}

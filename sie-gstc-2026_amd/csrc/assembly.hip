@@ -31,6 +31,7 @@
 // The inner q-loop therefore accumulates 4 complex moments per operator and
 // wavenumber (G, G b_q and g, g b_q) instead of 9 per operator.
 #include "sie_internal.hpp"
+#include "prof.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -40,6 +41,12 @@ namespace sie {
 namespace {
 
 constexpr double kPi = 3.14159265358979323846;
+// FP64 operations per point pair in far_pairs_kernel, counted on the source:
+// geometry (R, |R|^2, |R|, kR, weight scale) and, per wavenumber, G, g and the
+// 8 complex moment accumulations. sqrt/rsqrt and sincos are counted as one
+// operation each (their instruction sequences are not).
+constexpr double kFarGeomFlops = 3 + 5 + 1 + 2;
+constexpr double kFarPairFlops = 1 + 1 + 2 + 6 + 4 + 28;
 constexpr double kInv4Pi = 1.0 / (4.0 * kPi);
 
 struct cplx { double re, im; };
@@ -858,8 +865,12 @@ struct Assembler {
         SIE_HIP(hipMemcpyAsync(h_count, count.p, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
         // far pairs
         dim3 grid((nf + 63) / 64, (nc + 3) / 4);
+        prof().begin(PROF_FAR_PAIRS, s);
         far_pairs_kernel<NK><<<grid, 256, 0, s>>>(nf, n0, nc, Q, mesh->d_c.p, mesh->d_o.p, mesh->d_len.p, bq.p,
                                                   quad.xy.p, quad.w.p, k1, k2, thr, W.p);
+        // flops of the moment-factored point-pair update (kFarPairFlops per pair and wavenumber)
+        prof().end(PROF_FAR_PAIRS, s, (double)nf * nc * Q * Q * (kFarGeomFlops + NK * kFarPairFlops),
+                   (double)nf * nc * NK * 18 * 16.0);
         SIE_HIP(hipGetLastError());
         SIE_HIP(hipStreamSynchronize(s));
         unsigned int nn = *h_count;

@@ -20,6 +20,7 @@
 //  * getrs (op N / C), GEMV (op N / C) with deterministic two-stage
 //    reductions, and FP64 iterative refinement around an FP32 factorisation.
 #include "sie_internal.hpp"
+#include "prof.hpp"
 
 #include <hip/hip_cooperative_groups.h>
 
@@ -613,10 +614,13 @@ void gemm<float>(int M, int N, int K, double alpha, const c32* A, long long lda,
                  double beta, c32* C, long long ldc, hipStream_t s, int mode3m) {
     if (M <= 0 || N <= 0) return;
     dim3 grid((M + G32_BM - 1) / G32_BM, (N + G32_BN - 1) / G32_BN);
+    prof().begin(PROF_GEMM_F32, s);
     if (mode3m)
         cgemm_f32_kernel<1><<<grid, 256, 0, s>>>(M, N, K, (float)alpha, (float)beta, A, lda, B, ldb, C, ldc);
     else
         cgemm_f32_kernel<0><<<grid, 256, 0, s>>>(M, N, K, (float)alpha, (float)beta, A, lda, B, ldb, C, ldc);
+    // complex-equivalent flops 8MNK; bytes: A and B once, C read + written
+    prof().end(PROF_GEMM_F32, s, 8.0 * M * N * K, 8.0 * ((double)M * K + (double)K * N + 2.0 * M * N));
     SIE_HIP(hipGetLastError());
 }
 
@@ -625,7 +629,9 @@ void gemm<double>(int M, int N, int K, double alpha, const c64* A, long long lda
                   double beta, c64* C, long long ldc, hipStream_t s, int) {
     if (M <= 0 || N <= 0) return;
     dim3 grid((M + G64_BM - 1) / G64_BM, (N + G64_BN - 1) / G64_BN);
+    prof().begin(PROF_GEMM_F64, s);
     zgemm_f64_kernel<<<grid, 256, 0, s>>>(M, N, K, alpha, beta, A, lda, B, ldb, C, ldc);
+    prof().end(PROF_GEMM_F64, s, 8.0 * M * N * K, 16.0 * ((double)M * K + (double)K * N + 2.0 * M * N));
     SIE_HIP(hipGetLastError());
 }
 
@@ -684,8 +690,10 @@ static void panel(int M, int w, cx<T>* P, long long lda, int* ipiv, int* d_info,
     unsigned int* counter = (unsigned int*)(wsp + cand_b + pub_b + drow_b);
     SIE_HIP(hipMemsetAsync(counter, 0, sizeof(unsigned int), s));
     void* args[] = {&M, &w, &P, &lda, &ipiv, &cand, &pub, &drow, &counter, &d_info, &info_base};
+    prof().begin(PROF_PANEL, s);
     SIE_HIP(hipLaunchCooperativeKernel((const void*)panel_getf2_kernel<T, W, kPanelBS>, dim3(G), dim3(kPanelBS),
                                        args, 0, s));
+    prof().end(PROF_PANEL, s, 8.0 * M * w * w / 2, 2.0 * sizeof(cx<T>) * M * w);
 }
 
 // ipiv: relative panel indices -> absolute (add k1) for the swaps applied later.

@@ -97,3 +97,5 @@ _sig("sie_farfield", I, [P, I, D, D, P, I, P, P, P])
 _sig("sie_farfield_adjoint", I, [P, I, D, D, I, P, P, P, P])
 _sig("sie_gstc_mass_sensitivity", I, [P, I, I, Fl, Fl, P, P, P, P])
 _sig("sie_gstc_param_grad", I, [I, I, P, P, D, D, D, P, P])
+_sig("sie_profile_enable", I, [I, I])
+_sig("sie_profile_read", I, [I, ctypes.POINTER(I), ctypes.POINTER(D), ctypes.POINTER(D), ctypes.POINTER(D)])
