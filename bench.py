@@ -157,7 +157,7 @@ def main():
     # load every kernel once on a tiny problem (code objects, workspaces)
     tiny = sm.Mesh.icosphere(1.0, 1)
     tp = ad.GstcProblem(tiny, sm.GstcOptions(frequency_hz=o.frequency_hz, quadrature_rule=rule), waves, kind)
-    tst = tp.solve(torch.zeros((tiny.nf, kind), dtype=torch.float64) + 0.1 / k0)
+    tst = tp.solve(torch.from_numpy(np.random.default_rng(1).uniform(-2, 2, size=(tiny.nf, kind)) / k0))
     tJ, tF = tp.objective(tst)
     tp.gradient(tst, tF)
     del tst, tp

@@ -731,11 +731,15 @@ __global__ __launch_bounds__(64) void mass_kernel(
         }
 }
 
+// Grid-stride fill: the dispatch grid (blocks x threads) must stay below 2^32
+// work-items, so large matrices are covered by a capped grid.
 __global__ void fill_kernel(cplx* __restrict__ Z, long long ldz, int rows, int cols) {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long long)rows * cols) return;
-    const long long c = t / rows, r = t - c * rows;
-    Z[c * ldz + r] = {0, 0};
+    const long long tot = (long long)rows * cols;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < tot;
+         t += (long long)gridDim.x * blockDim.x) {
+        const long long c = t / rows, r = t - c * rows;
+        Z[c * ldz + r] = {0, 0};
+    }
 }
 
 // ------------------------------------------------------------------ RHS
@@ -929,7 +933,7 @@ static void assemble_generic(sie_mesh* h, const sie_calderon_params* p, const Ep
     if (A.W.n < wsz) A.W.alloc(wsz);
     {
         const long long tot = (long long)dim * dim;
-        fill_kernel<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(Z, ldz, dim, dim);
+        fill_kernel<<<(unsigned)std::min<long long>((tot + 255) / 256, 1 << 16), 256, 0, s>>>(Z, ldz, dim, dim);
         SIE_HIP(hipGetLastError());
     }
     std::vector<int> list;

@@ -549,14 +549,17 @@ __global__ void gemv_finish(int m, int nchunks, const cx<double>* __restrict__ p
 }
 
 // ============================================================= utilities
+// Grid-stride: the dispatch must stay below 2^32 work-items.
 template <typename TI, typename TO>
 __global__ void convert_kernel(long long rows, long long cols, const cx<TI>* __restrict__ A, long long lda,
                                cx<TO>* __restrict__ B, long long ldb, double scale) {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= rows * cols) return;
-    const long long c = t / rows, r = t - c * rows;
-    const cx<TI> v = A[c * lda + r];
-    B[c * ldb + r] = {(TO)(v.re * scale), (TO)(v.im * scale)};
+    const long long tot = rows * cols;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < tot;
+         t += (long long)gridDim.x * blockDim.x) {
+        const long long c = t / rows, r = t - c * rows;
+        const cx<TI> v = A[c * lda + r];
+        B[c * ldb + r] = {(TO)(v.re * scale), (TO)(v.im * scale)};
+    }
 }
 
 // partial sums of |x|^2 per block (deterministic two-stage)
@@ -927,7 +930,7 @@ int sie_convert(int from_precision, int to_precision, long long rows, long long 
     require(rows > 0 && cols > 0 && lda >= rows && ldb >= rows, "invalid conversion dimensions");
     hipStream_t s = (hipStream_t)stream;
     const long long t = rows * cols;
-    const unsigned g = (unsigned)((t + 255) / 256);
+    const unsigned g = (unsigned)std::min<long long>((t + 255) / 256, 1 << 16);
     if (from_precision == 64 && to_precision == 32)
         convert_kernel<double, float><<<g, 256, 0, s>>>(rows, cols, (const c64*)A, lda, (c32*)B, ldb, scale);
     else if (from_precision == 32 && to_precision == 64)

@@ -111,3 +111,16 @@ def test_rhs(sm, kind):
         ref = ol.oracle_rhs(mesh, k, o.quadrature_rule, w, o.vacuum_permeability, o.vacuum_permittivity,
                             o.vacuum_permeability, o.vacuum_permittivity)
         assert rel(B[:, c], ref) < 1e-13
+
+
+def test_assembly_overwrites_whole_matrix(sm):
+    """The matrix is written in full (zero-fill + gather) even when the output
+    buffer holds garbage, including past the first dispatch's reach."""
+    mesh = sm.Mesh.icosphere(1.0, 3)
+    o = gstc_opts(sm, rule=1)
+    t = torch.from_numpy(ol.random_gstc_tensors(mesh.nf, seed=4))
+    Z1 = sm.ops.assemble_gstc(mesh, o, t)
+    Z2 = sm.ops.colmajor(4 * mesh.ne)
+    Z2.fill_(complex(float("nan"), float("nan")))
+    sm.ops.assemble_gstc(mesh, o, t, out=Z2)
+    assert torch.equal(Z1, Z2)

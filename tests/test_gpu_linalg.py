@@ -114,3 +114,11 @@ def test_ir_solve(sm, op):
     xr = np.linalg.solve(M, b)
     assert np.all(rr < 1e-13)
     assert np.linalg.norm(X.cpu().numpy() - xr) / np.linalg.norm(xr) < 1e-12
+
+
+def test_convert_grid_stride(sm):
+    # more elements than one capped dispatch covers: exercises the grid-stride loop
+    n = 5000
+    A = torch.randn(n, n, dtype=torch.complex128, device="cuda").T
+    B = sm.linalg.to_precision(A, 32)
+    assert torch.equal(B, A.to(torch.complex64))
