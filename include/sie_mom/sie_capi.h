@@ -133,7 +133,9 @@ int sie_rhs_plane_waves(sie_mesh_t mesh, int kind, int rule, const sie_plane_wav
  * DenseLuSolver::solve takes (dense_lu_solver.cuh:122-166). */
 
 /* C = beta C + alpha A B, complex column-major (m x k times k x n) on the
- * matrix cores. variant (FP32 only): 0 = 4M real decomposition, 1 = 3M. */
+ * matrix cores. variant (FP32 only): 0 = 4M real decomposition, 1 = 3M,
+ * 2 = FP32-accurate split-FP16 (each FP32 operand split into FP16 hi + lo,
+ * three FP16 MFMAs per real product, FP32 accumulation). */
 int sie_gemm(int precision, int m, int n, int k, double alpha, const void* A_dev, long long lda,
              const void* B_dev, long long ldb, double beta, void* C_dev, long long ldc, int variant,
              void* stream);

@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 namespace sie {
 
@@ -246,6 +247,184 @@ __global__ __launch_bounds__(256, 2) void zgemm_f64_kernel(int M, int N, int K, 
                     c64 c = beta != 0.0 ? *p : c64{0.0, 0.0};
                     c.re = beta * c.re + alpha * acc[0][ti][tj][v];
                     c.im = beta * c.im + alpha * acc[1][ti][tj][v];
+                    *p = c;
+                }
+            }
+}
+
+// ===================================================== split-FP16 GEMM
+// C = beta C + alpha A B for complex FP32 operands on the FP16 matrix cores.
+// Each real operand x is split as x = h + l with h = fp16(x), l = fp16(x - h)
+// (22 significant bits), and every real product is formed from three FP16
+// MFMAs, h h' + h l' + l h' (the dropped l l' term is ~2^-22 relative), with
+// FP32 accumulation: FP32-level products at the FP16 MFMA rate
+// (v_mfma_f32_32x32x16_f16; lane l holds A[l%32][8(l/32)+e], e = 0..7, and the
+// 32x32x2f32 output layout). B is scaled by a power of two s (from its max
+// modulus) before splitting so that the low parts stay in the normal range;
+// the epilogue divides by s. Complex products use the 4M form.
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+
+// Split planes layout for an operand X with "row" index r (i for A, j for B)
+// and depth k: P[plane][k/16][r][16], plane = {re_h, re_l, im_h, im_l}.
+__global__ void maxabs_kernel(long long rows, long long cols, const c32* __restrict__ X, long long ld,
+                              unsigned int* __restrict__ out) {
+    float m = 0.f;
+    const long long tot = rows * cols;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (long long)gridDim.x * blockDim.x) {
+        const long long c = t / rows, r = t - c * rows;
+        const c32 v = X[c * ld + r];
+        m = fmaxf(m, fmaxf(fabsf(v.re), fabsf(v.im)));
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));  // non-negative floats order as uints
+}
+
+__device__ __forceinline__ float pow2_scale(const unsigned int* mx) {
+    const float m = __uint_as_float(*mx);
+    if (!(m > 0.f)) return 1.f;
+    int e;
+    frexpf(m, &e);  // m = f 2^e, f in [0.5, 1)
+    return ldexpf(1.f, -e);
+}
+
+// trans = 0: X is the A operand (M x K, element (r, k) at X[k*ld + r]);
+// trans = 1: X is the B operand (K x N, element (k, r) at X[r*ld + k]).
+__global__ void split_f16_kernel(int R, int K, const c32* __restrict__ X, long long ld, int trans,
+                                 const unsigned int* __restrict__ mx, _Float16* __restrict__ P) {
+    const long long tot = (long long)R * K;
+    const int KB = (K + 15) / 16;
+    const size_t plane = (size_t)KB * R * 16;
+    const float sc = mx ? pow2_scale(mx) : 1.f;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < (long long)KB * R * 16;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int kk = (int)(t & 15);
+        const long long rest = t >> 4;
+        const int r = (int)(rest % R);
+        const int kb = (int)(rest / R);
+        const int k = kb * 16 + kk;
+        c32 v = {0.f, 0.f};
+        if (k < K) v = trans ? X[(long long)r * ld + k] : X[(long long)k * ld + r];
+        v.re *= sc;
+        v.im *= sc;
+        const _Float16 rh = (_Float16)v.re, ih = (_Float16)v.im;
+        const _Float16 rl = (_Float16)(v.re - (float)rh), il = (_Float16)(v.im - (float)ih);
+        P[t] = rh;
+        P[plane + t] = rl;
+        P[2 * plane + t] = ih;
+        P[3 * plane + t] = il;
+    }
+    (void)tot;
+}
+
+constexpr int H_BM = 128, H_BN = 128;
+
+__device__ __forceinline__ half8 neg8(half8 x) {
+    half8 y;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) y[e] = -x[e];
+    return y;
+}
+
+__global__ __launch_bounds__(256, 2) void cgemm_f16x3_kernel(int M, int N, int K, float alpha, float beta,
+                                                             const _Float16* __restrict__ PA,
+                                                             const _Float16* __restrict__ PB,
+                                                             const unsigned int* __restrict__ mxB,
+                                                             c32* __restrict__ C, long long ldc) {
+    // sA[buf][plane][i][16], sB[buf][plane][j][16]
+    __shared__ half8 sA[2][4][H_BM][2];
+    __shared__ half8 sB[2][4][H_BN][2];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int i0 = blockIdx.x * H_BM, j0 = blockIdx.y * H_BN;
+    const int KB = (K + 15) / 16;
+    const size_t planeA = (size_t)KB * M * 16, planeB = (size_t)KB * N * 16;
+    floatx16 accR[2][2], accI[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) { accR[a][b][v] = 0.f; accI[a][b][v] = 0.f; }
+    // global -> register staging: A tile 4 planes x 128 rows x 2 half8 = 1024 half8; B same.
+    half8 ra[4], rb[4];
+    auto gload = [&](int kb) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int L = q * 256 + tid;          // 0..1023
+            const int plane = L >> 8, row = (L >> 1) & 127, part = L & 1;
+            const int gi = i0 + row, gj = j0 + row;
+            half8 za = {}, zb = {};
+            if (gi < M) za = *(const half8*)(PA + plane * planeA + ((size_t)kb * M + gi) * 16 + part * 8);
+            if (gj < N) zb = *(const half8*)(PB + plane * planeB + ((size_t)kb * N + gj) * 16 + part * 8);
+            ra[q] = za;
+            rb[q] = zb;
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int L = q * 256 + tid;
+            const int plane = L >> 8, row = (L >> 1) & 127, part = L & 1;
+            sA[buf][plane][row][part] = ra[q];
+            sB[buf][plane][row][part] = rb[q];
+        }
+    };
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    for (int kb = 0; kb < KB; ++kb) {
+        const int buf = kb & 1;
+        if (kb + 1 < KB) gload(kb + 1);
+        half8 a[2][4], b[2][4];
+        const int part = lane >> 5;
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int pl = 0; pl < 4; ++pl) {
+                a[t][pl] = sA[buf][pl][wm * 64 + t * 32 + (lane & 31)][part];
+                b[t][pl] = sB[buf][pl][wn * 64 + t * 32 + (lane & 31)][part];
+            }
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < 2; ++tj) {
+                const half8 brh = b[tj][0], brl = b[tj][1], bih = b[tj][2], bil = b[tj][3];
+                const half8 nbih = neg8(bih), nbil = neg8(bil);
+                const half8 arh = a[ti][0], arl = a[ti][1], aih = a[ti][2], ail = a[ti][3];
+                floatx16 cr = accR[ti][tj], ci = accI[ti][tj];
+                // small terms first
+                cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(brl, arh, cr, 0, 0, 0);
+                cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(brh, arl, cr, 0, 0, 0);
+                cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(nbil, aih, cr, 0, 0, 0);
+                cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(nbih, ail, cr, 0, 0, 0);
+                cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(brh, arh, cr, 0, 0, 0);
+                cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(nbih, aih, cr, 0, 0, 0);
+                ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(bil, arh, ci, 0, 0, 0);
+                ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(bih, arl, ci, 0, 0, 0);
+                ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(brl, aih, ci, 0, 0, 0);
+                ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(brh, ail, ci, 0, 0, 0);
+                ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(bih, arh, ci, 0, 0, 0);
+                ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(brh, aih, ci, 0, 0, 0);
+                accR[ti][tj] = cr;
+                accI[ti][tj] = ci;
+            }
+        if (kb + 1 < KB) lstore(buf ^ 1);
+        __syncthreads();
+    }
+    const float inv = alpha / pow2_scale(mxB);
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const long long i = i0 + wm * 64 + ti * 32 + (lane & 31);
+                const long long j = j0 + wn * 64 + tj * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                if (i < M && j < N) {
+                    c32* p = C + j * ldc + i;
+                    c32 c = beta != 0.f ? *p : c32{0.f, 0.f};
+                    c.re = beta * c.re + inv * accR[ti][tj][v];
+                    c.im = beta * c.im + inv * accI[ti][tj][v];
                     *p = c;
                 }
             }
@@ -593,6 +772,113 @@ __global__ void residual_kernel(long long n, const c64* __restrict__ b, c64* __r
     r[i] = {b[i].re - r[i].re, b[i].im - r[i].im};
 }
 
+// ================================================ single-RHS triangular solves
+// Blocked sweeps with TB-wide diagonal blocks whose inverses are formed once
+// per solve; each step is one launch that applies the block inverse to x_k
+// and updates every remaining row (memory-bound on the factor).
+constexpr int TB = 64;
+
+// inv[blk] (TB x TB, column-major) = inverse of diagonal block blk of the
+// unit-lower L (which = 0) or of the upper U (which = 1). One lane per column.
+template <typename T>
+__global__ void diag_inv_kernel(int n, const cx<T>* __restrict__ LU, long long lda, int which,
+                                cx<T>* __restrict__ inv) {
+    const int blk = blockIdx.x, c = threadIdx.x;
+    const int k0 = blk * TB;
+    const int w = min(TB, n - k0);
+    cx<T>* X = inv + (size_t)blk * TB * TB;
+    if (c >= TB) return;
+    auto A = [&](int i, int j) { return LU[(long long)(k0 + j) * lda + k0 + i]; };
+    for (int i = 0; i < TB; ++i) X[c * TB + i] = {0, 0};
+    if (c >= w) {
+        X[c * TB + c] = {1, 0};
+        return;
+    }
+    if (which == 0) {
+        X[c * TB + c] = {1, 0};
+        for (int i = c + 1; i < w; ++i) {
+            T sr = 0, si = 0;
+            for (int j = c; j < i; ++j) {
+                const cx<T> a = A(i, j), x = X[c * TB + j];
+                sr -= a.re * x.re - a.im * x.im;
+                si -= a.re * x.im + a.im * x.re;
+            }
+            X[c * TB + i] = {sr, si};
+        }
+    } else {
+        auto divd = [](cx<T> x, cx<T> d) {
+            const T q = d.re * d.re + d.im * d.im;
+            return cx<T>{(x.re * d.re + x.im * d.im) / q, (x.im * d.re - x.re * d.im) / q};
+        };
+        X[c * TB + c] = divd({1, 0}, A(c, c));
+        for (int i = c - 1; i >= 0; --i) {
+            T sr = 0, si = 0;
+            for (int j = i + 1; j <= c; ++j) {
+                const cx<T> a = A(i, j), x = X[c * TB + j];
+                sr -= a.re * x.re - a.im * x.im;
+                si -= a.re * x.im + a.im * x.re;
+            }
+            X[c * TB + i] = divd({sr, si}, A(i, i));
+        }
+    }
+}
+
+// One sweep step: y = M x_k (M = inv_k for op N, inv_k^H for op C), stored in
+// Y[k0:k0+w); rows r in [r0, r1) of b get b_r -= sum_c A(r, k0+c) y_c (op N)
+// or b_r -= sum_c conj(A(k0+c, r)) y_c (op C).
+template <typename T>
+__global__ __launch_bounds__(256) void trsv_step_kernel(int w, int k0, const cx<T>* __restrict__ LU, long long lda,
+                                                        const cx<T>* __restrict__ inv, int op,
+                                                        cx<T>* __restrict__ b, cx<T>* __restrict__ Y, int r0,
+                                                        int r1) {
+    __shared__ cx<T> sx[TB], sy[TB];
+    const int t = threadIdx.x;
+    if (t < TB) sx[t] = t < w ? b[k0 + t] : cx<T>{0, 0};
+    __syncthreads();
+    if (t < TB) {
+        T sr = 0, si = 0;
+        for (int j = 0; j < w; ++j) {
+            cx<T> m = op == 0 ? inv[j * TB + t] : inv[t * TB + j];
+            if (op != 0) m.im = -m.im;
+            sr += m.re * sx[j].re - m.im * sx[j].im;
+            si += m.re * sx[j].im + m.im * sx[j].re;
+        }
+        sy[t] = {sr, si};
+        if (blockIdx.x == 0 && t < w) Y[k0 + t] = sy[t];
+    }
+    __syncthreads();
+    const int r = r0 + blockIdx.x * blockDim.x + t;
+    if (r >= r1) return;
+    T sr = 0, si = 0;
+    if (op == 0) {
+        const cx<T>* a = LU + (long long)k0 * lda + r;
+        for (int c = 0; c < w; ++c, a += lda) {
+            const cx<T> v = *a;
+            sr += v.re * sy[c].re - v.im * sy[c].im;
+            si += v.re * sy[c].im + v.im * sy[c].re;
+        }
+    } else {
+        const cx<T>* a = LU + (long long)r * lda + k0;
+        for (int c = 0; c < w; ++c) {
+            const cx<T> v = a[c];
+            sr += v.re * sy[c].re + v.im * sy[c].im;
+            si += v.re * sy[c].im - v.im * sy[c].re;
+        }
+    }
+    b[r].re -= sr;
+    b[r].im -= si;
+}
+
+// out[i] = in[perm[i]] (gather) or out[perm[i]] = in[i] (scatter)
+template <typename T>
+__global__ void permute_kernel(int n, const int* __restrict__ perm, const cx<T>* __restrict__ in,
+                               cx<T>* __restrict__ out, int scatter) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (scatter) out[perm[i]] = in[i];
+    else out[i] = in[perm[i]];
+}
+
 }  // namespace
 
 // ==================================================================== host
@@ -604,7 +890,7 @@ struct Workspace {
     }
 };
 static Workspace& ws(int slot) {
-    static Workspace w[6];
+    static Workspace w[8];
     return w[slot];
 }
 
@@ -635,6 +921,31 @@ void gemm<double>(int M, int N, int K, double alpha, const c64* A, long long lda
     prof().begin(PROF_GEMM_F64, s);
     zgemm_f64_kernel<<<grid, 256, 0, s>>>(M, N, K, alpha, beta, A, lda, B, ldb, C, ldc);
     prof().end(PROF_GEMM_F64, s, 8.0 * M * N * K, 16.0 * ((double)M * K + (double)K * N + 2.0 * M * N));
+    SIE_HIP(hipGetLastError());
+}
+
+// FP32-complex GEMM on the FP16 matrix cores (split-FP16, see the kernel).
+static void gemm_f16x3(int M, int N, int K, double alpha, const c32* A, long long lda, const c32* B, long long ldb,
+                       double beta, c32* C, long long ldc, hipStream_t s) {
+    if (M <= 0 || N <= 0) return;
+    const int KB = (K + 15) / 16;
+    const size_t pa = (size_t)KB * M * 16 * 4, pb = (size_t)KB * N * 16 * 4;
+    unsigned char* w = (unsigned char*)ws(6).get((pa + pb) * sizeof(_Float16) + 256);
+    _Float16* PA = (_Float16*)w;
+    _Float16* PB = PA + pa;
+    unsigned int* mx = (unsigned int*)(w + (pa + pb) * sizeof(_Float16));
+    SIE_HIP(hipMemsetAsync(mx, 0, sizeof(unsigned int), s));
+    const long long tb = (long long)K * N;
+    maxabs_kernel<<<(unsigned)std::min<long long>((tb + 255) / 256, 4096), 256, 0, s>>>(K, N, B, ldb, mx);
+    const long long ta = (long long)KB * M * 16, tbb = (long long)KB * N * 16;
+    split_f16_kernel<<<(unsigned)std::min<long long>((ta + 255) / 256, 1 << 16), 256, 0, s>>>(M, K, A, lda, 0,
+                                                                                             nullptr, PA);
+    split_f16_kernel<<<(unsigned)std::min<long long>((tbb + 255) / 256, 1 << 16), 256, 0, s>>>(N, K, B, ldb, 1, mx,
+                                                                                              PB);
+    dim3 grid((M + H_BM - 1) / H_BM, (N + H_BN - 1) / H_BN);
+    prof().begin(PROF_GEMM_F32, s);
+    cgemm_f16x3_kernel<<<grid, 256, 0, s>>>(M, N, K, (float)alpha, (float)beta, PA, PB, mx, C, ldc);
+    prof().end(PROF_GEMM_F32, s, 8.0 * M * N * K, 8.0 * ((double)M * K + (double)K * N + 2.0 * M * N));
     SIE_HIP(hipGetLastError());
 }
 
@@ -749,6 +1060,14 @@ void getrf(int n, cx<T>* A, long long lda, int* ipiv, int* d_info, hipStream_t s
                             A + (long long)(k0 + nb) * lda + k1 + w, lda, s, 0);
             }
             // trailing update A22 -= L21 U12 (K = nb)
+            if constexpr (std::is_same<T, float>::value) {
+                if (mode3m == 2) {
+                    gemm_f16x3(n - k0 - nb, right, nb, -1.0, A + (long long)k0 * lda + k0 + nb, lda,
+                               A + (long long)(k0 + nb) * lda + k0, lda, 1.0,
+                               A + (long long)(k0 + nb) * lda + k0 + nb, lda, s);
+                    continue;
+                }
+            }
             gemm<T>(n - k0 - nb, right, nb, -1.0, A + (long long)k0 * lda + k0 + nb, lda,
                     A + (long long)(k0 + nb) * lda + k0, lda, 1.0, A + (long long)(k0 + nb) * lda + k0 + nb, lda, s,
                     mode3m);
@@ -756,11 +1075,56 @@ void getrf(int n, cx<T>* A, long long lda, int* ipiv, int* d_info, hipStream_t s
     }
 }
 
+// Single right-hand-side solve by TB-blocked sweeps (op 0 = N, 2 = C).
+template <typename T>
+void getrs_vec(int op, int n, const cx<T>* LU, long long lda, const int* ipiv, cx<T>* b, hipStream_t s) {
+    const int nb = (n + TB - 1) / TB;
+    const size_t invsz = (size_t)nb * TB * TB * sizeof(cx<T>);
+    unsigned char* w = (unsigned char*)ws(5).get(2 * invsz + 2 * (size_t)n * sizeof(cx<T>) + (size_t)n * sizeof(int));
+    cx<T>* invL = (cx<T>*)w;
+    cx<T>* invU = (cx<T>*)(w + invsz);
+    cx<T>* y = (cx<T>*)(w + 2 * invsz);
+    cx<T>* t = y + n;
+    int* perm = (int*)(t + n);
+    // the row permutation of P^T, composed on the host from the swap sequence
+    std::vector<int> hp(n), hpiv(n);
+    SIE_HIP(hipMemcpyAsync(hpiv.data(), ipiv, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    SIE_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < n; ++i) hp[i] = i;
+    for (int k = 0; k < n; ++k) std::swap(hp[k], hp[hpiv[k]]);
+    SIE_HIP(hipMemcpyAsync(perm, hp.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    diag_inv_kernel<T><<<nb, TB, 0, s>>>(n, LU, lda, 0, invL);
+    diag_inv_kernel<T><<<nb, TB, 0, s>>>(n, LU, lda, 1, invU);
+    const unsigned g = (unsigned)((n + 255) / 256);
+    auto step = [&](int k, const cx<T>* inv, int sop, cx<T>* vec, cx<T>* out, int r0, int r1) {
+        const int wdt = std::min(TB, n - k);
+        const int rows = std::max(0, r1 - r0);
+        const unsigned blocks = (unsigned)std::max(1, (rows + 255) / 256);
+        trsv_step_kernel<T><<<blocks, 256, 0, s>>>(wdt, k, LU, lda, inv + (size_t)(k / TB) * TB * TB, sop, vec, out,
+                                                  r0, r1);
+    };
+    if (op == 0) {
+        permute_kernel<T><<<g, 256, 0, s>>>(n, perm, b, t, 0);  // t = P^T b
+        for (int k = 0; k < n; k += TB) step(k, invL, 0, t, y, k + TB, n);            // L y = t
+        for (int k = (nb - 1) * TB; k >= 0; k -= TB) step(k, invU, 0, y, b, 0, k);    // U b = y
+    } else {
+        SIE_HIP(hipMemcpyAsync(t, b, sizeof(cx<T>) * n, hipMemcpyDeviceToDevice, s));
+        for (int k = 0; k < n; k += TB) step(k, invU, 2, t, y, k + TB, n);            // U^H y = b
+        for (int k = (nb - 1) * TB; k >= 0; k -= TB) step(k, invL, 2, y, t, 0, k);    // L^H t = y
+        permute_kernel<T><<<g, 256, 0, s>>>(n, perm, t, b, 1);  // b = P t
+    }
+    SIE_HIP(hipGetLastError());
+}
+
 // Solve op(A) X = B with the factors of getrf (op 0 = N, 2 = C).
 template <typename T>
 void getrs(int op, int n, int nrhs, const cx<T>* LU, long long lda, const int* ipiv, cx<T>* B, long long ldb,
            hipStream_t s) {
     constexpr int BS = 32;
+    if (nrhs <= 4) {
+        for (int c = 0; c < nrhs; ++c) getrs_vec<T>(op, n, LU, lda, ipiv, B + (long long)c * ldb, s);
+        return;
+    }
     if (op == 0) {
         laswp<T>(B, ldb, 0, nrhs, ipiv, 0, n, 0, false, s);
         // L y = b (unit lower), forward
@@ -873,7 +1237,9 @@ int sie_gemm(int precision, int m, int n, int k, double alpha, const void* A, lo
     require(precision == 32 || precision == 64, "precision must be 32 or 64");
     require(m >= 0 && n >= 0 && k >= 0 && lda >= std::max(1, m) && ldb >= std::max(1, k) && ldc >= std::max(1, m),
             "invalid GEMM dimensions");
-    if (precision == 32)
+    if (precision == 32 && variant == 2)
+        gemm_f16x3(m, n, k, alpha, (const c32*)A, lda, (const c32*)B, ldb, beta, (c32*)C, ldc, (hipStream_t)stream);
+    else if (precision == 32)
         gemm<float>(m, n, k, alpha, (const c32*)A, lda, (const c32*)B, ldb, beta, (c32*)C, ldc, (hipStream_t)stream,
                     variant);
     else

@@ -14,6 +14,8 @@ from . import _lib
 from .ops import _dptr, _stream, colmajor
 
 OPS = {"N": 0, "C": 2}
+# FP32 GEMM variants: 4M real decomposition, 3M, FP32-accurate split-FP16 (3 FP16 MFMAs per real product)
+VARIANTS = {"4m": 0, "3m": 1, "f16x3": 2}
 
 
 def _prec(t):
@@ -39,7 +41,7 @@ def gemm(A, B, C=None, alpha=1.0, beta=0.0, variant="4m"):
         C = colmajor(m, n, dtype=A.dtype)
         beta = 0.0
     _lib.check(_lib.lib.sie_gemm(_prec(A), m, n, k, float(alpha), _dptr(A), _ld(A), _dptr(B), _ld(B),
-                                 float(beta), _dptr(C), _ld(C), 1 if variant == "3m" else 0, _stream()))
+                                 float(beta), _dptr(C), _ld(C), VARIANTS[variant], _stream()))
     return C
 
 
@@ -49,6 +51,8 @@ class DenseLuSolver:
     def __init__(self, dimension=0, variant="4m"):
         if dimension < 0:
             raise ValueError("DenseLuSolver dimension must be positive")
+        if variant not in VARIANTS:
+            raise ValueError(f"unknown GEMM variant {variant!r}")
         self.dimension = dimension
         self.variant = variant
         self.pivots = None
@@ -62,7 +66,7 @@ class DenseLuSolver:
         self.pivots = torch.empty(n, dtype=torch.int32, device=A.device)
         info = ctypes.c_int(0)
         _lib.check(_lib.lib.sie_lu_factor(_prec(A), n, _dptr(A), _ld(A), _dptr(self.pivots), ctypes.byref(info),
-                                          1 if self.variant == "3m" else 0, _stream()))
+                                          VARIANTS[self.variant], _stream()))
         self.factors = A
         return A
 

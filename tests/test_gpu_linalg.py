@@ -24,7 +24,7 @@ def to_dev(a, dtype=torch.complex128):
     return t.T
 
 
-@pytest.mark.parametrize("prec,variant", [(64, "4m"), (32, "4m"), (32, "3m")])
+@pytest.mark.parametrize("prec,variant", [(64, "4m"), (32, "4m"), (32, "3m"), (32, "f16x3")])
 @pytest.mark.parametrize("m,n,k", [(128, 128, 16), (300, 200, 70), (513, 65, 256), (1, 1, 1), (1000, 3, 5)])
 def test_gemm(sm, prec, variant, m, n, k):
     rng = np.random.default_rng(m + n + k)
@@ -42,16 +42,16 @@ def well_conditioned(rng, n):
     return crandn(rng, n, n) + 2 * np.sqrt(n) * np.eye(n)
 
 
-@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("prec,variant", [(64, "4m"), (32, "4m"), (32, "f16x3")])
 @pytest.mark.parametrize("n", [16, 100, 300, 513, 1024])
-def test_lu_solve(sm, prec, n):
+def test_lu_solve(sm, prec, variant, n):
     rng = np.random.default_rng(n)
     A = crandn(rng, n, n)  # general (pivoting exercised)
     b = crandn(rng, n, 3)
     dt = torch.complex128 if prec == 64 else torch.complex64
     Ad = to_dev(A, dt)
-    print(f"lu prec={prec} n={n}", flush=True)
-    lu = sm.linalg.DenseLuSolver()
+    print(f"lu prec={prec} {variant} n={n}", flush=True)
+    lu = sm.linalg.DenseLuSolver(variant=variant)
     lu.factorize(Ad)
     for op in ("N", "C"):
         Bd = to_dev(b, dt)
@@ -100,14 +100,15 @@ def test_zgemv(sm, op):
 
 
 @pytest.mark.parametrize("op", ["N", "C"])
-def test_ir_solve(sm, op):
+@pytest.mark.parametrize("variant", ["4m", "f16x3"])
+def test_ir_solve(sm, op, variant):
     rng = np.random.default_rng(9)
     n = 700
     A = crandn(rng, n, n) + 5 * np.sqrt(n) * np.eye(n)
     b = crandn(rng, n, 2)
     Zd = to_dev(A)
     LU32 = sm.linalg.to_precision(Zd, 32)
-    lu = sm.linalg.DenseLuSolver()
+    lu = sm.linalg.DenseLuSolver(variant=variant)
     lu.factorize(LU32)
     X, rr, its = sm.linalg.ir_solve(Zd, LU32, lu.pivots, to_dev(b), op=op, tol=1e-13, maxit=20)
     M = A if op == "N" else A.conj().T
@@ -122,3 +123,17 @@ def test_convert_grid_stride(sm):
     A = torch.randn(n, n, dtype=torch.complex128, device="cuda").T
     B = sm.linalg.to_precision(A, 32)
     assert torch.equal(B, A.to(torch.complex64))
+
+
+@pytest.mark.parametrize("scale", [1e-30, 1.0, 1e30])
+def test_gemm_f16x3_scaling(sm, scale):
+    # the B operand is rescaled by a power of two before the FP16 split: results
+    # keep FP32-level accuracy far outside the FP16 range
+    rng = np.random.default_rng(4)
+    m, n, k = 200, 150, 96
+    A, B = crandn(rng, m, k), crandn(rng, k, n) * scale
+    dt = torch.complex64
+    Cd = sm.linalg.gemm(to_dev(A, dt), to_dev(B, dt), variant="f16x3")
+    ref = A @ B
+    err = np.abs(Cd.cpu().numpy() - ref).max() / (np.abs(A).max() * np.abs(B).max() * k)
+    assert err < 2e-7, err
