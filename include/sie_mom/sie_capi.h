@@ -225,7 +225,8 @@ int sie_gstc_param_grad(int nf, int kind, const double* records_dev, const doubl
 /* ------------------------------------------------------------ profiling */
 /* Per-kernel timing with HIP events on the launch stream (no counterpart in
  * the reference, which times phases with std::chrono). kernel: 0 FP32 GEMM,
- * 1 FP64 GEMM, 2 far-pair assembly, 3 LU panel. enable(k, 1) clears and
+ * 1 FP64 GEMM, 2 far-pair assembly, 3 LU panel, 4 near-pair assembly,
+ * 5 per-edge gather, 6 GEMV, 7 split-FP16 GEMM. enable(k, 1) clears and
  * starts recording; read() synchronises on the recorded events and returns
  * the launch count, summed duration and summed algorithmic flops / bytes. */
 int sie_profile_enable(int kernel, int on);

@@ -253,19 +253,24 @@ __global__ __launch_bounds__(256, 2) void zgemm_f64_kernel(int M, int N, int K, 
 }
 
 // ===================================================== split-FP16 GEMM
-// C = beta C + alpha A B for complex FP32 operands on the FP16 matrix cores.
-// Each real operand x is split as x = h + l with h = fp16(x), l = fp16(x - h)
-// (22 significant bits), and every real product is formed from three FP16
-// MFMAs, h h' + h l' + l h' (the dropped l l' term is ~2^-22 relative), with
-// FP32 accumulation: FP32-level products at the FP16 MFMA rate
-// (v_mfma_f32_32x32x16_f16; lane l holds A[l%32][8(l/32)+e], e = 0..7, and the
-// 32x32x2f32 output layout). B is scaled by a power of two s (from its max
-// modulus) before splitting so that the low parts stay in the normal range;
-// the epilogue divides by s. Complex products use the 4M form.
+// C = beta C + alpha A B for complex FP32 operands on the FP16 matrix cores
+// (v_mfma_f32_32x32x16_f16, FP32 accumulation). After power-of-two scaling of
+// each operand to max |re|, |im| < 1, every real entry x is split as
+//   h = fp16(x),  l' = fp16((x - h) 2^11)
+// (22 significant bits; the 2^11 offset keeps l' out of the FP16 subnormal
+// range down to |x| ~ 2^-13). A real product x y is then formed, scaled by
+// 2^11, from three FP16 products
+//   x y 2^11 ~ h_x (h_y 2^11) + h_x l'_y + l'_x h_y
+// (the dropped l l' term is ~2^-22 relative). B is stored as (h 2^11, l');
+// the kernel recovers h_y = (h_y 2^11) 2^-11 exactly in FP16. Complex products
+// use the 4M form. Lane layout of the FP16 MFMA: lane l holds X[l%32][8(l/32)+e]
+// and Y[8(l/32)+e][l%32], e = 0..7; the output follows the 32x32x2f32 layout.
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+constexpr float kLoShift = 2048.f;  // 2^11
 
-// Split planes layout for an operand X with "row" index r (i for A, j for B)
-// and depth k: P[plane][k/16][r][16], plane = {re_h, re_l, im_h, im_l}.
+// max(|re|, |im|) of a column-major rows x cols block, as the bits of a
+// non-negative float (which order like unsigned integers) in *out.
 __global__ void maxabs_kernel(long long rows, long long cols, const c32* __restrict__ X, long long ld,
                               unsigned int* __restrict__ out) {
     float m = 0.f;
@@ -276,158 +281,222 @@ __global__ void maxabs_kernel(long long rows, long long cols, const c32* __restr
         m = fmaxf(m, fmaxf(fabsf(v.re), fabsf(v.im)));
     }
     for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-    if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));  // non-negative floats order as uints
+    if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
 }
 
-__device__ __forceinline__ float pow2_scale(const unsigned int* mx) {
-    const float m = __uint_as_float(*mx);
-    if (!(m > 0.f)) return 1.f;
+// 2^-e with max = f 2^e, f in [0.5, 1): scaled entries have modulus < 1.
+__device__ __forceinline__ float pow2_scale(unsigned int bits) {
+    const float m = __uint_as_float(bits);
+    if (!(m > 0.f) || !(m < INFINITY)) return 1.f;
     int e;
-    frexpf(m, &e);  // m = f 2^e, f in [0.5, 1)
+    frexpf(m, &e);
     return ldexpf(1.f, -e);
 }
 
-// trans = 0: X is the A operand (M x K, element (r, k) at X[k*ld + r]);
-// trans = 1: X is the B operand (K x N, element (k, r) at X[r*ld + k]).
+// Split planes of an operand with "row" index r (i for A, j for B) and depth k:
+// P[plane][k/16][(k/8)%2][r][8], planes {re hi, re lo', im hi, im lo'}, depth
+// padded with zeros to a multiple of 16; the 8 halves at (k/16, (k/8)%2, r) are
+// one MFMA lane's operand. trans = 0: X is A (element (r, k) at X[k ld + r]);
+// trans = 1: X is B (element (k, r) at X[r ld + k]) and its hi planes carry the
+// 2^11 offset.
 __global__ void split_f16_kernel(int R, int K, const c32* __restrict__ X, long long ld, int trans,
                                  const unsigned int* __restrict__ mx, _Float16* __restrict__ P) {
-    const long long tot = (long long)R * K;
     const int KB = (K + 15) / 16;
-    const size_t plane = (size_t)KB * R * 16;
-    const float sc = mx ? pow2_scale(mx) : 1.f;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < (long long)KB * R * 16;
+    const long long total = (long long)KB * R * 16;
+    const size_t plane = (size_t)total;
+    const float sc = pow2_scale(*mx);
+    const float hs = trans ? kLoShift : 1.f;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
          t += (long long)gridDim.x * blockDim.x) {
-        const int kk = (int)(t & 15);
-        const long long rest = t >> 4;
+        const int e = (int)(t & 7);
+        const long long rest = t >> 3;
         const int r = (int)(rest % R);
-        const int kb = (int)(rest / R);
-        const int k = kb * 16 + kk;
+        const long long kp = rest / R;  // kb * 2 + part
+        const int k = (int)(kp * 8) + e;
         c32 v = {0.f, 0.f};
         if (k < K) v = trans ? X[(long long)r * ld + k] : X[(long long)k * ld + r];
-        v.re *= sc;
-        v.im *= sc;
-        const _Float16 rh = (_Float16)v.re, ih = (_Float16)v.im;
-        const _Float16 rl = (_Float16)(v.re - (float)rh), il = (_Float16)(v.im - (float)ih);
-        P[t] = rh;
-        P[plane + t] = rl;
-        P[2 * plane + t] = ih;
-        P[3 * plane + t] = il;
+        const float xr = v.re * sc, xi = v.im * sc;
+        const _Float16 rh = (_Float16)xr, ih = (_Float16)xi;
+        P[t] = (_Float16)((float)rh * hs);
+        P[plane + t] = (_Float16)((xr - (float)rh) * kLoShift);
+        P[2 * plane + t] = (_Float16)((float)ih * hs);
+        P[3 * plane + t] = (_Float16)((xi - (float)ih) * kLoShift);
     }
-    (void)tot;
 }
 
-constexpr int H_BM = 128, H_BN = 128;
+// Tile of a linear block id. Work-groups are dispatched round-robin over the
+// 8 XCDs, each with its own L2: give every XCD a contiguous range of tiles and
+// walk that range in groups of GM tile rows, column by column, so that the
+// blocks resident on one XCD share their A and B panels in its L2.
+template <bool SWZ>
+__device__ __forceinline__ void f16x3_tile(int id, int nbm, int nbn, int& tm, int& tn) {
+    if (!SWZ) {
+        tm = id % nbm;
+        tn = id / nbm;
+        return;
+    }
+    constexpr int GM = 8;
+    const int total = nbm * nbn;
+    const int per = total >> 3, rem = total & 7, x = id & 7;
+    const int lin = x * per + min(x, rem) + (id >> 3);
+    const int gsz = GM * nbn;
+    const int g = lin / gsz, first = g * GM, rows = min(nbm - first, GM);
+    const int r = lin - g * gsz;
+    tm = first + r % rows;
+    tn = r / rows;
+}
 
+typedef unsigned int uint4v __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ half8 neg8(half8 x) {
-    half8 y;
-#pragma unroll
-    for (int e = 0; e < 8; ++e) y[e] = -x[e];
-    return y;
+    // flip the eight sign bits, one 32-bit xor per pair of halves
+    return __builtin_bit_cast(half8, __builtin_bit_cast(uint4v, x) ^ 0x80008000u);
 }
+__device__ __forceinline__ half8 unshift8(half8 x) { return x * (_Float16)(1.f / kLoShift); }
 
-__global__ __launch_bounds__(256, 2) void cgemm_f16x3_kernel(int M, int N, int K, float alpha, float beta,
-                                                             const _Float16* __restrict__ PA,
-                                                             const _Float16* __restrict__ PB,
-                                                             const unsigned int* __restrict__ mxB,
-                                                             c32* __restrict__ C, long long ldc) {
-    // sA[buf][plane][i][16], sB[buf][plane][j][16]
-    __shared__ half8 sA[2][4][H_BM][2];
-    __shared__ half8 sB[2][4][H_BN][2];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
-    const int i0 = blockIdx.x * H_BM, j0 = blockIdx.y * H_BN;
-    const int KB = (K + 15) / 16;
-    const size_t planeA = (size_t)KB * M * 16, planeB = (size_t)KB * N * 16;
-    floatx16 accR[2][2], accI[2][2];
+// The accumulators start from beta C in accumulator units (C g with
+// g = beta 2^11 sA sB / alpha, exact for alpha, beta = +-1), loaded before the
+// first K slice so that the read of C overlaps that slice's loads and does
+// not stall the end of the tile; the epilogue then only writes C = f acc with
+// f = alpha / (2^11 sA sB). Requires |beta C g| < FLT_MAX.
+template <int TI>
+__device__ __forceinline__ void f16x3_acc_init(floatx16 (&accR)[TI][2], floatx16 (&accI)[TI][2],
+                                               const c32* __restrict__ C, long long ldc, int M, int N, int ib,
+                                               int jb, int r32, int part, float g, bool load) {
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int v = 0; v < 16; ++v) { accR[a][b][v] = 0.f; accI[a][b][v] = 0.f; }
-    // global -> register staging: A tile 4 planes x 128 rows x 2 half8 = 1024 half8; B same.
-    half8 ra[4], rb[4];
-    auto gload = [&](int kb) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int L = q * 256 + tid;          // 0..1023
-            const int plane = L >> 8, row = (L >> 1) & 127, part = L & 1;
-            const int gi = i0 + row, gj = j0 + row;
-            half8 za = {}, zb = {};
-            if (gi < M) za = *(const half8*)(PA + plane * planeA + ((size_t)kb * M + gi) * 16 + part * 8);
-            if (gj < N) zb = *(const half8*)(PB + plane * planeB + ((size_t)kb * N + gj) * 16 + part * 8);
-            ra[q] = za;
-            rb[q] = zb;
-        }
-    };
-    auto lstore = [&](int buf) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int L = q * 256 + tid;
-            const int plane = L >> 8, row = (L >> 1) & 127, part = L & 1;
-            sA[buf][plane][row][part] = ra[q];
-            sB[buf][plane][row][part] = rb[q];
-        }
-    };
-    gload(0);
-    lstore(0);
-    __syncthreads();
-    for (int kb = 0; kb < KB; ++kb) {
-        const int buf = kb & 1;
-        if (kb + 1 < KB) gload(kb + 1);
-        half8 a[2][4], b[2][4];
-        const int part = lane >> 5;
-#pragma unroll
-        for (int t = 0; t < 2; ++t)
-#pragma unroll
-            for (int pl = 0; pl < 4; ++pl) {
-                a[t][pl] = sA[buf][pl][wm * 64 + t * 32 + (lane & 31)][part];
-                b[t][pl] = sB[buf][pl][wn * 64 + t * 32 + (lane & 31)][part];
-            }
-#pragma unroll
-        for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-            for (int tj = 0; tj < 2; ++tj) {
-                const half8 brh = b[tj][0], brl = b[tj][1], bih = b[tj][2], bil = b[tj][3];
-                const half8 nbih = neg8(bih), nbil = neg8(bil);
-                const half8 arh = a[ti][0], arl = a[ti][1], aih = a[ti][2], ail = a[ti][3];
-                floatx16 cr = accR[ti][tj], ci = accI[ti][tj];
-                // small terms first
-                cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(brl, arh, cr, 0, 0, 0);
-                cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(brh, arl, cr, 0, 0, 0);
-                cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(nbil, aih, cr, 0, 0, 0);
-                cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(nbih, ail, cr, 0, 0, 0);
-                cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(brh, arh, cr, 0, 0, 0);
-                cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(nbih, aih, cr, 0, 0, 0);
-                ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(bil, arh, ci, 0, 0, 0);
-                ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(bih, arl, ci, 0, 0, 0);
-                ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(brl, aih, ci, 0, 0, 0);
-                ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(brh, ail, ci, 0, 0, 0);
-                ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(bih, arh, ci, 0, 0, 0);
-                ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(brh, aih, ci, 0, 0, 0);
-                accR[ti][tj] = cr;
-                accI[ti][tj] = ci;
-            }
-        if (kb + 1 < KB) lstore(buf ^ 1);
-        __syncthreads();
-    }
-    const float inv = alpha / pow2_scale(mxB);
-#pragma unroll
-    for (int ti = 0; ti < 2; ++ti)
+    for (int ti = 0; ti < TI; ++ti)
 #pragma unroll
         for (int tj = 0; tj < 2; ++tj)
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
-                const long long i = i0 + wm * 64 + ti * 32 + (lane & 31);
-                const long long j = j0 + wn * 64 + tj * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-                if (i < M && j < N) {
-                    c32* p = C + j * ldc + i;
-                    c32 c = beta != 0.f ? *p : c32{0.f, 0.f};
-                    c.re = beta * c.re + inv * accR[ti][tj][v];
-                    c.im = beta * c.im + inv * accI[ti][tj][v];
-                    *p = c;
-                }
+                const long long i = ib + ti * 32 + r32;
+                const long long j = jb + tj * 32 + 8 * (v >> 2) + 4 * part + (v & 3);
+                c32 c = {0.f, 0.f};
+                if (load && i < M && j < N) c = C[j * ldc + i];
+                accR[ti][tj][v] = c.re * g;
+                accI[ti][tj][v] = c.im * g;
             }
+}
+
+template <int TI>
+__device__ __forceinline__ void f16x3_acc_store(const floatx16 (&accR)[TI][2], const floatx16 (&accI)[TI][2],
+                                                c32* __restrict__ C, long long ldc, int M, int N, int ib, int jb,
+                                                int r32, int part, float f) {
+#pragma unroll
+    for (int ti = 0; ti < TI; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const long long i = ib + ti * 32 + r32;
+                const long long j = jb + tj * 32 + 8 * (v >> 2) + 4 * part + (v & 3);
+                if (i < M && j < N) C[j * ldc + i] = c32{f * accR[ti][tj][v], f * accI[ti][tj][v]};
+            }
+}
+
+#define SIE_GPTR(p) ((const void __attribute__((address_space(1)))*)(p))
+#define SIE_LPTR(p) ((void __attribute__((address_space(3)))*)(p))
+
+// Block tile (64 TI) x 128, four waves in a 2 x 2 grid, wave tile (32 TI) x 64.
+// K is consumed 16 at a time through two LDS buffers filled by direct
+// global->LDS loads (global_load_lds_dwordx4: lane q of a wave writes 16 bytes
+// at the wave's LDS base + 16 q). Each iteration waits for its own buffer,
+// reads its fragments, then starts the next slice's loads into the other
+// buffer so that they overlap this slice's MFMAs. Rows past M (columns past
+// N) load a clamped in-range row; they only feed outputs that are not stored.
+template <int TI, int MINB, bool SWZ>
+__global__ __launch_bounds__(256, MINB) void cgemm_f16x3_kernel(int M, int N, int K, float alpha, float beta,
+                                                                const _Float16* __restrict__ PA,
+                                                                const _Float16* __restrict__ PB,
+                                                                const unsigned int* __restrict__ mx,
+                                                                c32* __restrict__ C, long long ldc) {
+    constexpr int BM = 64 * TI, BN = 128;
+    constexpr int LA = BM * 8 / 256, LB = BN * 8 / 256;
+    // [buffer][plane][k half][row]: a lane's 16 bytes; 32 lanes read 512 contiguous bytes
+    __shared__ half8 sA[2][4][2][BM];
+    __shared__ half8 sB[2][4][2][BN];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    int tm, tn;
+    f16x3_tile<SWZ>(blockIdx.x, (M + BM - 1) / BM, (N + BN - 1) / BN, tm, tn);
+    const int i0 = tm * BM, j0 = tn * BN;
+    const int KB = (K + 15) / 16;
+    const size_t planeA = (size_t)KB * M * 16, planeB = (size_t)KB * N * 16;
+    // source offsets (halves) of this thread's 16-byte pieces in slice 0
+    size_t offA[LA], offB[LB];
+#pragma unroll
+    for (int q = 0; q < LA; ++q) {
+        const int L = q * 256 + tid;  // LDS piece (plane, k half, row), row fastest
+        const int plane = L / (2 * BM), part = (L / BM) & 1, row = L % BM;
+        offA[q] = plane * planeA + ((size_t)part * M + min(i0 + row, M - 1)) * 8;
+    }
+#pragma unroll
+    for (int q = 0; q < LB; ++q) {
+        const int L = q * 256 + tid;
+        const int plane = L / (2 * BN), part = (L / BN) & 1, row = L % BN;
+        offB[q] = plane * planeB + ((size_t)part * N + min(j0 + row, N - 1)) * 8;
+    }
+    auto dma = [&](int kb, int buf) {
+        const size_t ka = (size_t)kb * M * 16, kbb = (size_t)kb * N * 16;
+        half8* la = &sA[buf][0][0][0] + wave * 64;
+        half8* lb = &sB[buf][0][0][0] + wave * 64;  // lane q lands at base + 16 q bytes
+#pragma unroll
+        for (int q = 0; q < LA; ++q)
+            __builtin_amdgcn_global_load_lds(SIE_GPTR(PA + offA[q] + ka), SIE_LPTR(la + q * 256), 16, 0, 0);
+#pragma unroll
+        for (int q = 0; q < LB; ++q)
+            __builtin_amdgcn_global_load_lds(SIE_GPTR(PB + offB[q] + kbb), SIE_LPTR(lb + q * 256), 16, 0, 0);
+    };
+    floatx16 accR[TI][2], accI[TI][2];
+    const float unit = pow2_scale(mx[0]) * pow2_scale(mx[1]) * kLoShift;
+    f16x3_acc_init<TI>(accR, accI, C, ldc, M, N, i0 + wm * 32 * TI, j0 + wn * 64, lane & 31, lane >> 5,
+                       beta * unit / alpha, beta != 0.f);
+    dma(0, 0);
+    const int part = lane >> 5, r32 = lane & 31;
+    for (int kb = 0; kb < KB; ++kb) {
+        const int buf = kb & 1;
+        // this slice's loads (the only ones in flight) have landed for every wave,
+        // and every wave is done reading the other buffer
+        asm volatile("s_waitcnt vmcnt(0)\n\ts_barrier" ::: "memory");
+        half8 a[TI][4], b[2][4];
+#pragma unroll
+        for (int t = 0; t < TI; ++t)
+#pragma unroll
+            for (int pl = 0; pl < 4; ++pl) a[t][pl] = sA[buf][pl][part][wm * 32 * TI + t * 32 + r32];
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int pl = 0; pl < 4; ++pl) b[t][pl] = sB[buf][pl][part][wn * 64 + t * 32 + r32];
+        if (kb + 1 < KB) dma(kb + 1, buf ^ 1);
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj) {
+            // B hi planes carry 2^11: b?2 = h 2^11, b?l = l 2^11, b? = h
+            const half8 br2 = b[tj][0], brl = b[tj][1], bi2 = b[tj][2], bil = b[tj][3];
+            const half8 br = unshift8(br2), bi = unshift8(bi2);
+#pragma unroll
+            for (int ti = 0; ti < TI; ++ti) {
+                const half8 arh = a[ti][0], arl = a[ti][1], aih = a[ti][2], ail = a[ti][3];
+                const half8 naih = neg8(aih), nail = neg8(ail);
+                floatx16 cr = accR[ti][tj], ci = accI[ti][tj];
+                // re: ar br - ai bi, small terms first
+                cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(brl, arh, cr, 0, 0, 0);
+                cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(br, arl, cr, 0, 0, 0);
+                cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(bil, naih, cr, 0, 0, 0);
+                cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(bi, nail, cr, 0, 0, 0);
+                cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(br2, arh, cr, 0, 0, 0);
+                cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(bi2, naih, cr, 0, 0, 0);
+                // im: ar bi + ai br
+                ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(bil, arh, ci, 0, 0, 0);
+                ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(bi, arl, ci, 0, 0, 0);
+                ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(brl, aih, ci, 0, 0, 0);
+                ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(br, ail, ci, 0, 0, 0);
+                ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(bi2, arh, ci, 0, 0, 0);
+                ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(br2, aih, ci, 0, 0, 0);
+                accR[ti][tj] = cr;
+                accI[ti][tj] = ci;
+            }
+        }
+    }
+    f16x3_acc_store<TI>(accR, accI, C, ldc, M, N, i0 + wm * 32 * TI, j0 + wn * 64, lane & 31, lane >> 5, alpha / unit);
 }
 
 // =================================================================== panel
@@ -928,24 +997,33 @@ void gemm<double>(int M, int N, int K, double alpha, const c64* A, long long lda
 static void gemm_f16x3(int M, int N, int K, double alpha, const c32* A, long long lda, const c32* B, long long ldb,
                        double beta, c32* C, long long ldc, hipStream_t s) {
     if (M <= 0 || N <= 0) return;
+    if (K <= 0 || alpha == 0.0) {  // C = beta C
+        gemm<float>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s, 0);
+        return;
+    }
     const int KB = (K + 15) / 16;
     const size_t pa = (size_t)KB * M * 16 * 4, pb = (size_t)KB * N * 16 * 4;
     unsigned char* w = (unsigned char*)ws(6).get((pa + pb) * sizeof(_Float16) + 256);
     _Float16* PA = (_Float16*)w;
     _Float16* PB = PA + pa;
     unsigned int* mx = (unsigned int*)(w + (pa + pb) * sizeof(_Float16));
-    SIE_HIP(hipMemsetAsync(mx, 0, sizeof(unsigned int), s));
-    const long long tb = (long long)K * N;
-    maxabs_kernel<<<(unsigned)std::min<long long>((tb + 255) / 256, 4096), 256, 0, s>>>(K, N, B, ldb, mx);
-    const long long ta = (long long)KB * M * 16, tbb = (long long)KB * N * 16;
-    split_f16_kernel<<<(unsigned)std::min<long long>((ta + 255) / 256, 1 << 16), 256, 0, s>>>(M, K, A, lda, 0,
-                                                                                             nullptr, PA);
-    split_f16_kernel<<<(unsigned)std::min<long long>((tbb + 255) / 256, 1 << 16), 256, 0, s>>>(N, K, B, ldb, 1, mx,
-                                                                                              PB);
-    dim3 grid((M + H_BM - 1) / H_BM, (N + H_BN - 1) / H_BN);
-    prof().begin(PROF_GEMM_F32, s);
-    cgemm_f16x3_kernel<<<grid, 256, 0, s>>>(M, N, K, (float)alpha, (float)beta, PA, PB, mx, C, ldc);
-    prof().end(PROF_GEMM_F32, s, 8.0 * M * N * K, 8.0 * ((double)M * K + (double)K * N + 2.0 * M * N));
+    SIE_HIP(hipMemsetAsync(mx, 0, 2 * sizeof(unsigned int), s));
+    const long long ta = (long long)M * K, tb = (long long)K * N;
+    maxabs_kernel<<<(unsigned)std::min<long long>((ta + 255) / 256, 4096), 256, 0, s>>>(M, K, A, lda, mx);
+    maxabs_kernel<<<(unsigned)std::min<long long>((tb + 255) / 256, 4096), 256, 0, s>>>(K, N, B, ldb, mx + 1);
+    const long long sa = (long long)KB * M * 16, sb = (long long)KB * N * 16;
+    split_f16_kernel<<<(unsigned)std::min<long long>((sa + 255) / 256, 1 << 16), 256, 0, s>>>(M, K, A, lda, 0, mx,
+                                                                                             PA);
+    split_f16_kernel<<<(unsigned)std::min<long long>((sb + 255) / 256, 1 << 16), 256, 0, s>>>(N, K, B, ldb, 1,
+                                                                                             mx + 1, PB);
+    const long long tiles = (long long)((M + 127) / 128) * ((N + 127) / 128);
+    require(tiles * 256 < (1LL << 32), "split-FP16 GEMM: too many tiles for one dispatch");
+    prof().begin(PROF_GEMM_F16X3, s);
+    cgemm_f16x3_kernel<2, 2, true><<<(unsigned)tiles, 256, 0, s>>>(M, N, K, (float)alpha, (float)beta, PA, PB, mx, C,
+                                                                   ldc);
+    // algorithmic (complex-equivalent) flops 8MNK; the kernel issues 3x that on
+    // the FP16 matrix cores. bytes: split A and B planes once, C read + written
+    prof().end(PROF_GEMM_F16X3, s, 8.0 * M * N * K, 8.0 * ((double)M * K + (double)K * N) + 16.0 * M * N);
     SIE_HIP(hipGetLastError());
 }
 
