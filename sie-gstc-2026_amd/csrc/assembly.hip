@@ -32,6 +32,8 @@
 // wavenumber (G, G b_q and g, g b_q) instead of 9 per operator.
 #include "sie_internal.hpp"
 #include "prof.hpp"
+#define SIE_HD __host__ __device__
+#include "trig.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -151,7 +153,7 @@ __global__ __launch_bounds__(256) void far_pairs_kernel(
 #pragma unroll
             for (int w = 0; w < NK; ++w) {
                 double s, co;
-                sincos(kk[w] * Rm, &s, &co);
+                sincos_moderate(kk[w] * Rm, &s, &co);
                 // w_q G = w_q e^{-jkR} / (4 pi R)
                 const double Gr = co * gsc, Gi = -s * gsc;
                 // w_q g = w_q G (1/R + jk) / R   (grad'G = g R_vec)
@@ -359,7 +361,7 @@ __global__ __launch_bounds__(128) void near_pairs_kernel(
                 } else {
                     const double iR = 1.0 / Rm, iR2 = iR * iR, iR3 = iR2 * iR;
                     double s, co;
-                    sincos(kk[w] * Rm, &s, &co);
+                    sincos_moderate(kk[w] * Rm, &s, &co);
                     G[w] = {(co - 1.0) * kInv4Pi * iR, -s * kInv4Pi * iR};
                     // K_reg = (1/R^3 + jk/R^2) e^{-jkR} - (1/R^3 + k^2/(2R))
                     double Kr = iR3 * co + kk[w] * iR2 * s;
