@@ -128,9 +128,10 @@ int sie_rhs_plane_waves(sie_mesh_t mesh, int kind, int rule, const sie_plane_wav
 
 /* ------------------------------------------------------------ dense algebra */
 /* precision: 64 = complex double (interleaved doubles), 32 = complex float.
- * op: 0 = N, 2 = C (conjugate transpose); the numbering follows
- * cublasOperation_t (CUBLAS_OP_N = 0, CUBLAS_OP_C = 2), which the reference's
- * DenseLuSolver::solve takes (dense_lu_solver.cuh:122-166). */
+ * op: 0 = N, 1 = T (transpose), 2 = C (conjugate transpose); the numbering
+ * follows cublasOperation_t (CUBLAS_OP_N = 0, CUBLAS_OP_T = 1, CUBLAS_OP_C = 2),
+ * which the reference's DenseLuSolver::solve takes (dense_lu_solver.cuh:122-166).
+ * sie_lu_solve accepts all three; GEMV and iterative refinement take N and C. */
 
 /* C = beta C + alpha A B, complex column-major (m x k times k x n) on the
  * matrix cores. variant (FP32 only): 0 = 4M real decomposition, 1 = 3M,
@@ -151,7 +152,8 @@ int sie_lu_factor(int precision, int n, void* A_dev, long long lda, int* ipiv_de
 
 /* Solve op(A) X = B in place with the factors of sie_lu_factor; B is n x nrhs
  * (ldb). Replaces DenseLuSolver::solve / cusolverDnZgetrs
- * (dense_lu_solver.cuh:122-166); op = 2 is the adjoint solve with A^H. */
+ * (dense_lu_solver.cuh:122-166); op = 2 is the adjoint solve with A^H and
+ * op = 1 (A^T) is solved as A^H conj(X) = conj(B). */
 int sie_lu_solve(int precision, int op, int n, int nrhs, const void* LU_dev, long long lda,
                  const int* ipiv_dev, void* B_dev, long long ldb, void* stream);
 
@@ -179,6 +181,31 @@ int sie_ir_solve(int op, int n, int nrhs, const double* Z_dev, long long ldz, co
 /* Euclidean norm of n complex doubles (deterministic two-stage reduction,
  * synchronises the stream; returns -1 on error). */
 double sie_norm2(long long n, const double* x_dev, void* stream);
+
+/* ------------------------------------------------------------- GMRES */
+/* y_dev = A x_dev for an operator supplied by the caller, enqueued on stream;
+ * returns SIE_OK or an SIE_ERR_* code (the solve stops with that code). */
+typedef int (*sie_matvec_fn)(void* user, const double* x_dev, double* y_dev, void* stream);
+
+/* Restarted GMRES(m) for A x = b, complex double, replacing gmres_restarted /
+ * GMRESSolver::solve (include/sie_mom/cuda/gmres/gmres_solver.cuh:258-537) with
+ * its contract: x_dev holds the initial guess on entry; tol is relative to |b|;
+ * |b| < 1e-20 gives x = 0; *out_res is the absolute residual estimate of the
+ * last Arnoldi step; *out_iters counts inner steps over all restarts; *info is
+ * 0 when converged, 1 when maxnit restarts did not reach tol; history (may be
+ * NULL) receives the relative residual after each inner step, up to
+ * history_cap values, *history_len the number written. Arnoldi by classical
+ * Gram-Schmidt with one reorthogonalisation (two GEMV passes over the basis);
+ * device workspace n x (m+1) complex. Synchronises the stream. */
+int sie_gmres(int n, int m, int maxnit, double tol, sie_matvec_fn matvec, void* user, const double* b_dev,
+              double* x_dev, double* out_res, int* out_iters, int* info, double* history, int history_cap,
+              int* history_len, void* stream);
+
+/* sie_gmres with the stored-matrix operator y = Z x (Z column-major, ldz):
+ * the stored-Z counterpart of the reference's matvec_Z_* kernels. */
+int sie_gmres_dense(int n, const double* Z_dev, long long ldz, int m, int maxnit, double tol,
+                    const double* b_dev, double* x_dev, double* out_res, int* out_iters, int* info,
+                    double* history, int history_cap, int* history_len, void* stream);
 
 /* ----------------------------------------------------------- adjoint */
 /* GSTC surface maps from real per-face parameters (kind = parameters per

@@ -1194,11 +1194,34 @@ void getrs_vec(int op, int n, const cx<T>* LU, long long lda, const int* ipiv, c
     SIE_HIP(hipGetLastError());
 }
 
-// Solve op(A) X = B with the factors of getrf (op 0 = N, 2 = C).
+// B = conj(B) on an n x nrhs block.
+template <typename T>
+__global__ void conj_kernel(int n, int nrhs, cx<T>* __restrict__ B, long long ldb) {
+    const long long tot = (long long)n * nrhs;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (long long)gridDim.x * blockDim.x) {
+        const long long c = t / n, r = t - c * n;
+        B[c * ldb + r].im = -B[c * ldb + r].im;
+    }
+}
+
+template <typename T>
+void getrs(int op, int n, int nrhs, const cx<T>* LU, long long lda, const int* ipiv, cx<T>* B, long long ldb,
+           hipStream_t s);
+
+// Solve op(A) X = B with the factors of getrf (op 0 = N, 1 = T, 2 = C).
+// A^T X = B is solved as A^H conj(X) = conj(B).
 template <typename T>
 void getrs(int op, int n, int nrhs, const cx<T>* LU, long long lda, const int* ipiv, cx<T>* B, long long ldb,
            hipStream_t s) {
     constexpr int BS = 32;
+    if (op == 1) {
+        const unsigned g = (unsigned)std::min<long long>(((long long)n * nrhs + 255) / 256, 1 << 16);
+        conj_kernel<T><<<g, 256, 0, s>>>(n, nrhs, B, ldb);
+        getrs<T>(2, n, nrhs, LU, lda, ipiv, B, ldb, s);
+        conj_kernel<T><<<g, 256, 0, s>>>(n, nrhs, B, ldb);
+        SIE_HIP(hipGetLastError());
+        return;
+    }
     if (nrhs <= 4) {
         for (int c = 0; c < nrhs; ++c) getrs_vec<T>(op, n, LU, lda, ipiv, B + (long long)c * ldb, s);
         return;
@@ -1349,7 +1372,7 @@ int sie_lu_solve(int precision, int op, int n, int nrhs, const void* LU, long lo
                  long long ldb, void* stream) {
     GUARD_BEGIN
     require(precision == 32 || precision == 64, "precision must be 32 or 64");
-    require(op == 0 || op == 2, "op must be 0 (N) or 2 (C)");
+    require(op == 0 || op == 1 || op == 2, "op must be 0 (N), 1 (T) or 2 (C)");
     require(n > 0 && nrhs > 0 && lda >= n && ldb >= n, "invalid solve dimensions");
     hipStream_t s = (hipStream_t)stream;
     if (precision == 32)

@@ -29,6 +29,10 @@ inline void require(bool ok, const std::string& msg) {
 
 void set_last_error(const std::string& m);
 
+int gmres_run(int n, int m, int maxnit, double tol, sie_matvec_fn matvec, void* user, const double* b, double* x,
+              double* out_res, int* out_iters, double* history, int history_cap, int* history_len,
+              hipStream_t s);
+
 // ------------------------------------------------------------ device memory
 template <typename T>
 struct DevBuf {

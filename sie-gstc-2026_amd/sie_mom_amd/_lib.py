@@ -92,6 +92,11 @@ _sig("sie_zgemv", I, [I, I, I, D, P, LL, P, D, P, P])
 _sig("sie_convert", I, [I, I, LL, LL, P, LL, P, LL, D, P])
 _sig("sie_ir_solve", I, [I, I, I, P, LL, P, LL, P, P, LL, P, LL, D, I, P, P, P])
 _sig("sie_norm2", D, [LL, P, P])
+MATVEC_FN = ctypes.CFUNCTYPE(I, P, P, P, P)
+_sig("sie_gmres", I, [I, I, I, D, MATVEC_FN, P, P, P, ctypes.POINTER(D), ctypes.POINTER(I), ctypes.POINTER(I), P, I,
+                      ctypes.POINTER(I), P])
+_sig("sie_gmres_dense", I, [I, P, LL, I, I, D, P, P, ctypes.POINTER(D), ctypes.POINTER(I), ctypes.POINTER(I), P, I,
+                            ctypes.POINTER(I), P])
 _sig("sie_gstc_records", I, [I, I, P, D, D, D, D, P, ctypes.POINTER(I), P])
 _sig("sie_farfield", I, [P, I, D, D, P, I, P, P, P])
 _sig("sie_farfield_adjoint", I, [P, I, D, D, I, P, P, P, P])

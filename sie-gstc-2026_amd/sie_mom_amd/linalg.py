@@ -13,7 +13,7 @@ import torch
 from . import _lib
 from .ops import _dptr, _stream, colmajor
 
-OPS = {"N": 0, "C": 2}
+OPS = {"N": 0, "T": 1, "C": 2}
 # FP32 GEMM variants: 4M real decomposition, 3M, FP32-accurate split-FP16 (3 FP16 MFMAs per real product)
 VARIANTS = {"4m": 0, "3m": 1, "f16x3": 2}
 
@@ -117,3 +117,79 @@ def ir_solve(Z, LU32, pivots, B, op="N", tol=1e-12, maxit=30):
                                      float(tol), int(maxit), rr.ctypes.data_as(ctypes.c_void_p),
                                      its.ctypes.data_as(ctypes.c_void_p), _stream()))
     return (X if B.dim() == 2 else X[:, 0]), rr, its
+
+
+class _DeviceVector:
+    """Zero-copy view of n complex doubles at a device address (for torch.as_tensor)."""
+
+    def __init__(self, ptr, n):
+        self.__cuda_array_interface__ = {"shape": (n,), "typestr": "<c16", "data": (ptr, False), "version": 3,
+                                         "strides": None}
+
+
+class GmresResult:
+    def __init__(self, x, info, residual, iterations, history):
+        self.x, self.info, self.residual, self.iterations, self.history = x, info, residual, iterations, history
+
+    @property
+    def converged(self):
+        return self.info == 0
+
+
+def _gmres_call(n, rhs, x, m, maxnit, tol, runner):
+    hist = np.zeros(m * maxnit, dtype=np.float64)
+    res, iters, info, hlen = ctypes.c_double(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    runner(res, iters, info, hist, hlen)
+    return GmresResult(x, info.value, res.value, iters.value, hist[:hlen.value].copy())
+
+
+def gmres(Z, b, x0=None, m=50, maxnit=20, tol=1e-8):
+    """Restarted GMRES(m) on a stored complex128 column-major matrix Z (the
+    stored-Z matvec of gmres_restarted, gmres_solver.cuh:258-520)."""
+    n = Z.shape[0]
+    if Z.dtype != torch.complex128 or b.dtype != torch.complex128:
+        raise TypeError("complex128 operator and right-hand side expected")
+    bb = b.contiguous()
+    x = torch.zeros_like(bb) if x0 is None else x0.clone().contiguous()
+
+    def run(res, iters, info, hist, hlen):
+        _lib.check(_lib.lib.sie_gmres_dense(n, _dptr(Z), _ld(Z), m, maxnit, float(tol), _dptr(bb), _dptr(x),
+                                            ctypes.byref(res), ctypes.byref(iters), ctypes.byref(info),
+                                            hist.ctypes.data_as(ctypes.c_void_p), hist.size, ctypes.byref(hlen),
+                                            _stream()))
+    return _gmres_call(n, bb, x, m, maxnit, tol, run)
+
+
+class GMRESSolver:
+    """GMRESSolver<MatVecOp> (gmres_solver.cuh:524-537): matvec(x, y) writes
+    y = A x for complex128 device vectors of length neqns."""
+
+    def __init__(self, neqns, matvec):
+        if neqns <= 0:
+            raise ValueError("GMRES dimension must be positive")
+        self.neqns, self.matvec = neqns, matvec
+
+    def solve(self, rhs, x, m=50, maxnit=20, tol=1e-8):
+        n = self.neqns
+        errors = []
+
+        def cb(user, xp, yp, stream):
+            try:
+                xv = torch.as_tensor(_DeviceVector(xp, n), device="cuda")
+                yv = torch.as_tensor(_DeviceVector(yp, n), device="cuda")
+                self.matvec(xv, yv)
+                return _lib.SIE_OK
+            except Exception as e:  # reported after the C call returns
+                errors.append(e)
+                return _lib.SIE_ERR_INVALID
+        fn = _lib.MATVEC_FN(cb)
+        bb = rhs.contiguous()
+
+        def run(res, iters, info, hist, hlen):
+            code = _lib.lib.sie_gmres(n, m, maxnit, float(tol), fn, None, _dptr(bb), _dptr(x), ctypes.byref(res),
+                                      ctypes.byref(iters), ctypes.byref(info), hist.ctypes.data_as(ctypes.c_void_p),
+                                      hist.size, ctypes.byref(hlen), _stream())
+            if errors:
+                raise errors[0]
+            _lib.check(code)
+        return _gmres_call(n, bb, x, m, maxnit, tol, run)

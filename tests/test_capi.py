@@ -37,3 +37,17 @@ def test_invalid_arguments_rejected_without_gpu():
     f = _lib.lib.sie_icosphere
     f.argtypes = [ctypes.c_float, ctypes.c_int] + [ctypes.c_void_p] * 5
     assert f(1.0, -1, None, None, None, None, counts) == _lib.SIE_ERR_INVALID
+
+
+def test_gmres_validation_without_gpu():
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "sie-gstc-2026_amd"))
+    from sie_mom_amd import _lib
+    res, it, info, hl = ctypes.c_double(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    rc = _lib.lib.sie_gmres_dense(10, None, 10, 5, 5, 1e-8, None, None, ctypes.byref(res), ctypes.byref(it),
+                                  ctypes.byref(info), None, 0, ctypes.byref(hl), None)
+    assert rc == _lib.SIE_ERR_INVALID
+    rc = _lib.lib.sie_gmres(10, 0, 5, 1e-8, _lib.MATVEC_FN(lambda *a: 0), None, None, None, ctypes.byref(res),
+                            ctypes.byref(it), ctypes.byref(info), None, 0, ctypes.byref(hl), None)
+    assert rc == _lib.SIE_ERR_INVALID
+    assert "restart" in _lib.lib.sie_last_error().decode()

@@ -47,17 +47,17 @@ def well_conditioned(rng, n):
 def test_lu_solve(sm, prec, variant, n):
     rng = np.random.default_rng(n)
     A = crandn(rng, n, n)  # general (pivoting exercised)
-    b = crandn(rng, n, 3)
+    b = crandn(rng, n, 3)  # 3 columns: the single-RHS sweep path
     dt = torch.complex128 if prec == 64 else torch.complex64
     Ad = to_dev(A, dt)
     print(f"lu prec={prec} {variant} n={n}", flush=True)
     lu = sm.linalg.DenseLuSolver(variant=variant)
     lu.factorize(Ad)
-    for op in ("N", "C"):
+    for op in ("N", "T", "C"):
         Bd = to_dev(b, dt)
         lu.solve(Bd, op=op)
         X = Bd.cpu().numpy()
-        M = A if op == "N" else A.conj().T
+        M = {"N": A, "T": A.T, "C": A.conj().T}[op]
         res = np.linalg.norm(M @ X - b) / (np.linalg.norm(M) * np.linalg.norm(X))
         assert res < (1e-14 if prec == 64 else 5e-6), (op, res)
     # factors reproduce P A = L U
@@ -137,3 +137,74 @@ def test_gemm_f16x3_scaling(sm, scale):
     ref = A @ B
     err = np.abs(Cd.cpu().numpy() - ref).max() / (np.abs(A).max() * np.abs(B).max() * k)
     assert err < 2e-7, err
+
+
+@pytest.mark.parametrize("op", ["N", "T", "C"])
+def test_lu_solve_many_rhs(sm, op):
+    # more than 4 right-hand sides: the blocked TRSM path
+    rng = np.random.default_rng(21)
+    n, nrhs = 700, 9
+    A = crandn(rng, n, n)
+    b = crandn(rng, n, nrhs)
+    Ad = to_dev(A)
+    lu = sm.linalg.DenseLuSolver()
+    lu.factorize(Ad)
+    Bd = to_dev(b)
+    lu.solve(Bd, op=op)
+    M = {"N": A, "T": A.T, "C": A.conj().T}[op]
+    X = Bd.cpu().numpy()
+    assert np.linalg.norm(M @ X - b) / (np.linalg.norm(M) * np.linalg.norm(X)) < 1e-14
+
+
+def _gmres_system(rng, n, shift):
+    # complex Gaussian entries: eigenvalues in a disk of radius ~sqrt(2n) about shift sqrt(n)
+    A = crandn(rng, n, n) + shift * np.sqrt(n) * np.eye(n)
+    return A, crandn(rng, n)
+
+
+def test_gmres_dense_matches_lu(sm):
+    rng = np.random.default_rng(31)
+    n = 400
+    A, b = _gmres_system(rng, n, 2.0)
+    r = sm.linalg.gmres(to_dev(A), torch.from_numpy(b).cuda(), m=40, maxnit=10, tol=1e-12)
+    xr = np.linalg.solve(A, b)
+    assert r.converged and r.iterations > 0
+    assert np.linalg.norm(r.x.cpu().numpy() - xr) / np.linalg.norm(xr) < 1e-10
+    assert r.residual <= 1e-12 * np.linalg.norm(b) * (1 + 1e-9)
+    assert len(r.history) == r.iterations and r.history[-1] <= 1e-12
+
+
+def test_gmres_restarts_and_callback(sm):
+    rng = np.random.default_rng(32)
+    n = 300
+    # spectrum in a disk of radius ~sqrt(2n) about 2.5 sqrt(n): away from the origin,
+    # so GMRES(8) converges, but needs several restarts for 1e-10
+    A, b = _gmres_system(rng, n, 2.5)
+    Ad = to_dev(A)
+
+    def matvec(x, y):
+        y.copy_(Ad @ x)
+    x = torch.zeros(n, dtype=torch.complex128, device="cuda")
+    r = sm.linalg.GMRESSolver(n, matvec).solve(torch.from_numpy(b).cuda(), x, m=8, maxnit=400, tol=1e-10)
+    xr = np.linalg.solve(A, b)
+    assert r.converged and r.iterations > 8, (r.info, r.iterations, r.history[-5:])  # needed restarts
+    assert np.linalg.norm(x.cpu().numpy() - xr) / np.linalg.norm(xr) < 1e-8
+
+
+def test_gmres_edge_cases(sm):
+    rng = np.random.default_rng(33)
+    n = 64
+    A, b = _gmres_system(rng, n, 2.0)
+    Ad = to_dev(A)
+    # zero right-hand side: x = 0 without iterations
+    r = sm.linalg.gmres(Ad, torch.zeros(n, dtype=torch.complex128, device="cuda"), x0=torch.ones(n, dtype=torch.complex128, device="cuda"))
+    assert r.converged and r.iterations == 0 and r.residual == 0 and torch.all(r.x == 0)
+    # exact initial guess: converged at the first residual check
+    xr = torch.from_numpy(np.linalg.solve(A, b)).cuda()
+    r = sm.linalg.gmres(Ad, torch.from_numpy(b).cuda(), x0=xr, tol=1e-8)
+    assert r.converged and r.iterations == 0
+    # too few steps: not converged, info 1
+    r = sm.linalg.gmres(Ad, torch.from_numpy(b).cuda(), m=2, maxnit=1, tol=1e-14)
+    assert r.info == 1 and r.iterations == 2
+    with pytest.raises(ValueError):
+        sm.linalg.gmres(Ad, torch.from_numpy(b).cuda(), m=0)
