@@ -10,7 +10,7 @@ FLAGS=(-O3 -std=c++17 -fPIC --offload-arch=gfx950 -Xclang -target-feature -Xclan
 TMP="$HERE/_build"
 mkdir -p "$TMP"
 pids=()
-for f in "$SRC"/*.hip "$SRC"/*.cpp; do
+for f in "$SRC"/*.hip "$SRC"/*.cpp "$SRC"/host/*.cpp; do
   b=$(basename "$f")
   hipcc "${FLAGS[@]}" -c -x hip "$f" -o "$TMP/$b.o" &
   pids+=($!)
@@ -20,3 +20,11 @@ for p in "${pids[@]}"; do wait "$p"; done
 hipcc -shared --offload-arch=gfx950 -o "$OUT.tmp" "$TMP"/*.o
 mv "$OUT.tmp" "$OUT"
 echo "built $OUT"
+# the C++ interface check program used by tests/test_cpp_interface.py (a plain
+# C++17 caller of include/sie_mom/*.hpp, linked against the library)
+CHK="$HERE/../tests/cpp/sie_cpp_check"
+g++ -std=c++17 -O2 -D__HIP_PLATFORM_AMD__ -I"$HERE/../include" -I/opt/rocm/include "$CHK.cpp" \
+    -L"$HERE/sie_mom_amd" -lsie_amd -L/opt/rocm/lib -lamdhip64 \
+    -Wl,-rpath,'$ORIGIN/../../sie-gstc-2026_amd/sie_mom_amd' -Wl,-rpath,/opt/rocm/lib -o "$CHK.tmp"
+mv "$CHK.tmp" "$CHK"
+echo "built $CHK"
