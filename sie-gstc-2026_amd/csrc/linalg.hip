@@ -271,8 +271,9 @@ constexpr float kLoShift = 2048.f;  // 2^11
 
 // max(|re|, |im|) of a column-major rows x cols block, as the bits of a
 // non-negative float (which order like unsigned integers) in *out.
-__global__ void maxabs_kernel(long long rows, long long cols, const c32* __restrict__ X, long long ld,
-                              unsigned int* __restrict__ out) {
+__global__ __launch_bounds__(256) void maxabs_kernel(long long rows, long long cols, const c32* __restrict__ X,
+                                                     long long ld, unsigned int* __restrict__ out) {
+    __shared__ float wmax[4];
     float m = 0.f;
     const long long tot = rows * cols;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (long long)gridDim.x * blockDim.x) {
@@ -281,7 +282,10 @@ __global__ void maxabs_kernel(long long rows, long long cols, const c32* __restr
         m = fmaxf(m, fmaxf(fabsf(v.re), fabsf(v.im)));
     }
     for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-    if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    // one atomic per block: non-negative floats order like their bit patterns
+    if (threadIdx.x == 0) atomicMax(out, __float_as_uint(fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]))));
 }
 
 // 2^-e with max = f 2^e, f in [0.5, 1): scaled entries have modulus < 1.
@@ -1009,8 +1013,8 @@ static void gemm_f16x3(int M, int N, int K, double alpha, const c32* A, long lon
     unsigned int* mx = (unsigned int*)(w + (pa + pb) * sizeof(_Float16));
     SIE_HIP(hipMemsetAsync(mx, 0, 2 * sizeof(unsigned int), s));
     const long long ta = (long long)M * K, tb = (long long)K * N;
-    maxabs_kernel<<<(unsigned)std::min<long long>((ta + 255) / 256, 4096), 256, 0, s>>>(M, K, A, lda, mx);
-    maxabs_kernel<<<(unsigned)std::min<long long>((tb + 255) / 256, 4096), 256, 0, s>>>(K, N, B, ldb, mx + 1);
+    maxabs_kernel<<<(unsigned)std::min<long long>((ta + 255) / 256, 1024), 256, 0, s>>>(M, K, A, lda, mx);
+    maxabs_kernel<<<(unsigned)std::min<long long>((tb + 255) / 256, 1024), 256, 0, s>>>(K, N, B, ldb, mx + 1);
     const long long sa = (long long)KB * M * 16, sb = (long long)KB * N * 16;
     split_f16_kernel<<<(unsigned)std::min<long long>((sa + 255) / 256, 1 << 16), 256, 0, s>>>(M, K, A, lda, 0, mx,
                                                                                              PA);
