@@ -1098,61 +1098,103 @@ __global__ void offset_ipiv(int* ipiv, int n, int off) {
     if (i < n) ipiv[i] += off;
 }
 
-constexpr int kNB = 256;
+constexpr int kNB = 256;   // block of the right-looking steps inside a super-panel
+constexpr int kSP = 1024;  // super-panel: the trailing matrix is updated once per kSP columns
 
-// Blocked right-looking LU with partial pivoting; ipiv is 0-based absolute.
+// C -= L U (M x N, K) on the matrix cores: the split-FP16 GEMM for variant 2
+// in FP32, else the FP32 (4M / 3M) or FP64 MFMA kernels.
 template <typename T>
-void getrf(int n, cx<T>* A, long long lda, int* ipiv, int* d_info, hipStream_t s, int mode3m) {
+static void update(int M, int N, int K, const cx<T>* L, const cx<T>* U, cx<T>* C, long long lda, hipStream_t s,
+                   int variant) {
+    if (M <= 0 || N <= 0 || K <= 0) return;
+    if constexpr (std::is_same<T, float>::value) {
+        if (variant == 2) {
+            gemm_f16x3(M, N, K, -1.0, L, lda, U, lda, 1.0, C, lda, s);
+            return;
+        }
+    }
+    gemm<T>(M, N, K, -1.0, L, lda, U, lda, 1.0, C, lda, s, variant == 1 ? 1 : 0);
+}
+
+// U12 = L11^{-1} A12 for the unit lower triangle L11 at rows/columns
+// [r0, r0 + h) and the columns [c0, c0 + nc), in W-row blocks with
+// NB-blocked updates of the rows below.
+template <typename T>
+static void trsm_lower(cx<T>* A, long long lda, int r0, int h, int c0, int nc, hipStream_t s, int variant) {
     constexpr int W = PanelCfg<T>::W;
-    SIE_HIP(hipMemsetAsync(d_info, 0, sizeof(int), s));
-    for (int k0 = 0; k0 < n; k0 += kNB) {
-        const int nb = std::min(kNB, n - k0);
-        // factor the outer panel A[k0:n, k0:k0+nb] in W-wide sub-panels
+    if (nc <= 0) return;
+    for (int kb = r0; kb < r0 + h; kb += kNB) {
+        const int nb = std::min(kNB, r0 + h - kb);
+        for (int k1 = kb; k1 < kb + nb; k1 += W) {
+            const int w = std::min(W, kb + nb - k1);
+            cx<T>* B12 = A + (long long)c0 * lda + k1;
+            trsm_small<T>(w, A + (long long)k1 * lda + k1, lda, B12, lda, nc, 0, s);
+            const int below = kb + nb - (k1 + w);
+            if (below > 0)
+                gemm<T>(below, nc, w, -1.0, A + (long long)k1 * lda + k1 + w, lda, B12, lda, 1.0,
+                        A + (long long)c0 * lda + k1 + w, lda, s, 0);
+        }
+        // rows of the triangle below this NB block
+        update<T>(r0 + h - (kb + nb), nc, nb, A + (long long)kb * lda + kb + nb, A + (long long)c0 * lda + kb,
+                  A + (long long)c0 * lda + kb + nb, lda, s, variant);
+    }
+}
+
+// Factors the column block [c0, c1) over rows [c0, n) by right-looking NB
+// steps of W-wide cooperative panels; row swaps are applied inside the block only.
+template <typename T>
+static void getrf_block(int n, cx<T>* A, long long lda, int c0, int c1, int* ipiv, int* d_info, hipStream_t s,
+                        int variant) {
+    constexpr int W = PanelCfg<T>::W;
+    for (int k0 = c0; k0 < c1; k0 += kNB) {
+        const int nb = std::min(kNB, c1 - k0);
+        // the NB panel A[k0:n, k0:k0+nb) in W-wide sub-panels
         for (int k1 = k0; k1 < k0 + nb; k1 += W) {
             const int w = std::min(W, k0 + nb - k1);
             cx<T>* P = A + (long long)k1 * lda + k1;
             panel<T>(n - k1, w, P, lda, ipiv + k1, d_info, k1, s);
             offset_ipiv<<<1, W, 0, s>>>(ipiv + k1, w, k1);
-            // swaps inside the outer panel: columns [k0, k1) and [k1+w, k0+nb)
             laswp<T>(A, lda, k0, k1 - k0, ipiv, k1, k1 + w, 0, false, s);
             laswp<T>(A, lda, k1 + w, k0 + nb - (k1 + w), ipiv, k1, k1 + w, 0, false, s);
             const int rest = k0 + nb - (k1 + w);
             if (rest > 0) {
-                // U12 of the sub-panel
                 trsm_small<T>(w, P, lda, A + (long long)(k1 + w) * lda + k1, lda, rest, 0, s);
-                // trailing update within the outer panel
                 gemm<T>(n - k1 - w, rest, w, -1.0, A + (long long)k1 * lda + k1 + w, lda,
                         A + (long long)(k1 + w) * lda + k1, lda, 1.0, A + (long long)(k1 + w) * lda + k1 + w, lda, s,
                         0);
             }
         }
-        // swaps of this outer panel on the columns left and right of it
-        laswp<T>(A, lda, 0, k0, ipiv, k0, k0 + nb, 0, false, s);
-        const int right = n - k0 - nb;
+        // this NB panel's swaps on the other columns of the block
+        laswp<T>(A, lda, c0, k0 - c0, ipiv, k0, k0 + nb, 0, false, s);
+        const int right = c1 - k0 - nb;
         if (right > 0) {
             laswp<T>(A, lda, k0 + nb, right, ipiv, k0, k0 + nb, 0, false, s);
-            // U12 = L11^{-1} A12, in W-row blocks
-            for (int k1 = k0; k1 < k0 + nb; k1 += W) {
-                const int w = std::min(W, k0 + nb - k1);
-                cx<T>* B12 = A + (long long)(k0 + nb) * lda + k1;
-                trsm_small<T>(w, A + (long long)k1 * lda + k1, lda, B12, lda, right, 0, s);
-                const int below = k0 + nb - (k1 + w);
-                if (below > 0)
-                    gemm<T>(below, right, w, -1.0, A + (long long)k1 * lda + k1 + w, lda, B12, lda, 1.0,
-                            A + (long long)(k0 + nb) * lda + k1 + w, lda, s, 0);
-            }
-            // trailing update A22 -= L21 U12 (K = nb)
-            if constexpr (std::is_same<T, float>::value) {
-                if (mode3m == 2) {
-                    gemm_f16x3(n - k0 - nb, right, nb, -1.0, A + (long long)k0 * lda + k0 + nb, lda,
-                               A + (long long)(k0 + nb) * lda + k0, lda, 1.0,
-                               A + (long long)(k0 + nb) * lda + k0 + nb, lda, s);
-                    continue;
-                }
-            }
-            gemm<T>(n - k0 - nb, right, nb, -1.0, A + (long long)k0 * lda + k0 + nb, lda,
-                    A + (long long)(k0 + nb) * lda + k0, lda, 1.0, A + (long long)(k0 + nb) * lda + k0 + nb, lda, s,
-                    mode3m);
+            trsm_lower<T>(A, lda, k0, nb, k0 + nb, right, s, variant);
+            update<T>(n - k0 - nb, right, nb, A + (long long)k0 * lda + k0 + nb, A + (long long)(k0 + nb) * lda + k0,
+                      A + (long long)(k0 + nb) * lda + k0 + nb, lda, s, variant);
+        }
+    }
+}
+
+// Blocked right-looking LU with partial pivoting; ipiv is 0-based absolute.
+// Two levels: each super-panel of kSP columns is factored by NB steps that
+// update only its own columns; the rest of the matrix then gets the
+// super-panel's swaps, its U12 and one trailing update with K = kSP, so the
+// trailing matrix is read and written once per kSP columns instead of once
+// per NB columns (the update is otherwise HBM-bound at K = NB).
+template <typename T>
+void getrf(int n, cx<T>* A, long long lda, int* ipiv, int* d_info, hipStream_t s, int variant) {
+    SIE_HIP(hipMemsetAsync(d_info, 0, sizeof(int), s));
+    for (int K0 = 0; K0 < n; K0 += kSP) {
+        const int sp = std::min(kSP, n - K0);
+        getrf_block<T>(n, A, lda, K0, K0 + sp, ipiv, d_info, s, variant);
+        laswp<T>(A, lda, 0, K0, ipiv, K0, K0 + sp, 0, false, s);
+        const int right = n - K0 - sp;
+        if (right > 0) {
+            laswp<T>(A, lda, K0 + sp, right, ipiv, K0, K0 + sp, 0, false, s);
+            trsm_lower<T>(A, lda, K0, sp, K0 + sp, right, s, variant);
+            update<T>(n - K0 - sp, right, sp, A + (long long)K0 * lda + K0 + sp, A + (long long)(K0 + sp) * lda + K0,
+                      A + (long long)(K0 + sp) * lda + K0 + sp, lda, s, variant);
         }
     }
 }
