@@ -25,6 +25,7 @@
 #include <hip/hip_cooperative_groups.h>
 
 #include <algorithm>
+#include <map>
 #include <cmath>
 #include <type_traits>
 
@@ -509,9 +510,11 @@ __device__ __forceinline__ void grid_barrier(unsigned int* counter, unsigned int
     __syncthreads();
     if (threadIdx.x == 0) {
         __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        // relaxed polling, then one acquire fence
+        while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
             __builtin_amdgcn_s_sleep(1);
         }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
 }
@@ -853,93 +856,189 @@ constexpr int TB = 64;
 
 // inv[blk] (TB x TB, column-major) = inverse of diagonal block blk of the
 // unit-lower L (which = 0) or of the upper U (which = 1). One lane per column.
-template <typename T>
+// Inverse of every B x B diagonal block of the unit lower (which = 0) or upper
+// (which = 1) factor, one thread per column; inv[blk][c B + i] = X(i, c).
+// Partial last blocks are padded with the identity.
+template <typename T, int B>
 __global__ void diag_inv_kernel(int n, const cx<T>* __restrict__ LU, long long lda, int which,
                                 cx<T>* __restrict__ inv) {
     const int blk = blockIdx.x, c = threadIdx.x;
-    const int k0 = blk * TB;
-    const int w = min(TB, n - k0);
-    cx<T>* X = inv + (size_t)blk * TB * TB;
-    if (c >= TB) return;
+    const int k0 = blk * B;
+    const int w = min(B, n - k0);
+    cx<T>* X = inv + (size_t)blk * B * B;
+    if (c >= B) return;
     auto A = [&](int i, int j) { return LU[(long long)(k0 + j) * lda + k0 + i]; };
-    for (int i = 0; i < TB; ++i) X[c * TB + i] = {0, 0};
+    for (int i = 0; i < B; ++i) X[c * B + i] = {0, 0};
     if (c >= w) {
-        X[c * TB + c] = {1, 0};
+        X[c * B + c] = {1, 0};
         return;
     }
     if (which == 0) {
-        X[c * TB + c] = {1, 0};
+        X[c * B + c] = {1, 0};
         for (int i = c + 1; i < w; ++i) {
             T sr = 0, si = 0;
             for (int j = c; j < i; ++j) {
-                const cx<T> a = A(i, j), x = X[c * TB + j];
+                const cx<T> a = A(i, j), x = X[c * B + j];
                 sr -= a.re * x.re - a.im * x.im;
                 si -= a.re * x.im + a.im * x.re;
             }
-            X[c * TB + i] = {sr, si};
+            X[c * B + i] = {sr, si};
         }
     } else {
         auto divd = [](cx<T> x, cx<T> d) {
             const T q = d.re * d.re + d.im * d.im;
             return cx<T>{(x.re * d.re + x.im * d.im) / q, (x.im * d.re - x.re * d.im) / q};
         };
-        X[c * TB + c] = divd({1, 0}, A(c, c));
+        X[c * B + c] = divd({1, 0}, A(c, c));
         for (int i = c - 1; i >= 0; --i) {
             T sr = 0, si = 0;
             for (int j = i + 1; j <= c; ++j) {
-                const cx<T> a = A(i, j), x = X[c * TB + j];
+                const cx<T> a = A(i, j), x = X[c * B + j];
                 sr -= a.re * x.re - a.im * x.im;
                 si -= a.re * x.im + a.im * x.re;
             }
-            X[c * TB + i] = divd({sr, si}, A(i, i));
+            X[c * B + i] = divd({sr, si}, A(i, i));
         }
     }
 }
 
-// One sweep step: y = M x_k (M = inv_k for op N, inv_k^H for op C), stored in
-// Y[k0:k0+w); rows r in [r0, r1) of b get b_r -= sum_c A(r, k0+c) y_c (op N)
-// or b_r -= sum_c conj(A(k0+c, r)) y_c (op C).
-template <typename T>
-__global__ __launch_bounds__(256) void trsv_step_kernel(int w, int k0, const cx<T>* __restrict__ LU, long long lda,
-                                                        const cx<T>* __restrict__ inv, int op,
-                                                        cx<T>* __restrict__ b, cx<T>* __restrict__ Y, int r0,
-                                                        int r1) {
-    __shared__ cx<T> sx[TB], sy[TB];
-    const int t = threadIdx.x;
-    if (t < TB) sx[t] = t < w ? b[k0 + t] : cx<T>{0, 0};
-    __syncthreads();
-    if (t < TB) {
-        T sr = 0, si = 0;
-        for (int j = 0; j < w; ++j) {
-            cx<T> m = op == 0 ? inv[j * TB + t] : inv[t * TB + j];
-            if (op != 0) m.im = -m.im;
-            sr += m.re * sx[j].re - m.im * sx[j].im;
-            si += m.re * sx[j].im + m.im * sx[j].re;
-        }
-        sy[t] = {sr, si};
-        if (blockIdx.x == 0 && t < w) Y[k0 + t] = sy[t];
+// Sweep step of a single-RHS block triangular solve with B-row diagonal
+// blocks, with the diagonal solve of the next block fused in. Y holds the
+// solution of block k (columns [k0, k0 + w)). Every workgroup updates B rows
+// rb.. of vec in [r0, r1):
+//   op N: vec[r] -= sum_c A(r, k0 + c) y_c        (A(r, c) = LU[c lda + r])
+//   op C: vec[r] -= sum_c conj(A(k0 + c, r)) y_c   (a contiguous run of column r)
+// and the workgroup whose rows start at nxt0 (the next diagonal block, nw
+// rows) then solves it: out[nxt0 + i] = sum_j M(i, j) vec[nxt0 + j] with M the
+// block's precomputed inverse (op N) or its conjugate transpose (op C).
+template <typename T, int B>
+__device__ __forceinline__ void trsv_block_solve(const cx<T>* __restrict__ M, const cx<T>* v, int nw, T (*spr)[B],
+                                                 T (*spi)[B], cx<T>* __restrict__ out) {
+    // out[i] = sum_j M[j B + i] v_j: M is the inverse (op N) or its conjugate
+    // transpose (op C), column-major, so each load is B consecutive entries
+    constexpr int P = 256 / B, J = B / P;
+    const int t = threadIdx.x, i = t % B, part = t / B;
+    T ar = 0, ai = 0;
+    // no early exit: every load is independent and in bounds (M is a full
+    // B x B block), so the loop pipelines; entries past nw contribute zero
+#pragma unroll 16
+    for (int jj = 0; jj < J; ++jj) {
+        const int j = part * J + jj;
+        const cx<T> x = j < nw ? v[j] : cx<T>{0, 0};
+        const cx<T> m = M[j * B + i];
+        ar += m.re * x.re - m.im * x.im;
+        ai += m.re * x.im + m.im * x.re;
     }
+    spr[part][i] = ar;
+    spi[part][i] = ai;
     __syncthreads();
-    const int r = r0 + blockIdx.x * blockDim.x + t;
-    if (r >= r1) return;
-    T sr = 0, si = 0;
+    if (t < nw) {
+        T re = 0, im = 0;
+#pragma unroll
+        for (int q = 0; q < P; ++q) { re += spr[q][t]; im += spi[q][t]; }
+        out[t] = cx<T>{re, im};
+    }
+}
+
+// Mh = M^H for every B x B block: Mh[blk][j B + i] = conj(M[blk][i B + j]).
+template <typename T, int B>
+__global__ void conj_transpose_blocks_kernel(long long total, const cx<T>* __restrict__ M, cx<T>* __restrict__ Mh) {
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long long)gridDim.x * blockDim.x) {
+        const long long blk = e / (B * B);
+        const int r = (int)(e % (B * B)), j = r / B, i = r % B;
+        const cx<T> m = M[blk * B * B + (long long)i * B + j];
+        Mh[e] = cx<T>{m.re, -m.im};
+    }
+}
+
+template <typename T, int B>
+__global__ __launch_bounds__(256) void trsv_update_kernel(int w, int k0, const cx<T>* __restrict__ LU, long long lda,
+                                                          const cx<T>* __restrict__ Y, int op, cx<T>* __restrict__ vec,
+                                                          int r0, int r1, int nxt0, int nw,
+                                                          const cx<T>* __restrict__ invn, cx<T>* __restrict__ out) {
+    constexpr int P = 256 / B;        // op N: column groups
+    constexpr int CG = B / P;         // columns per group
+    constexpr int L = B / 4;          // op C: lanes per row, 4 entries each
+    constexpr int RP = 256 / L;       // op C: rows per pass
+    __shared__ cx<T> sy[B];
+    __shared__ cx<T> sres[B];
+    __shared__ T spr[P][B], spi[P][B];
+    const int t = threadIdx.x;
+    const int rb = r0 + blockIdx.x * B;
+    if (t < B) sy[t] = t < w ? Y[k0 + t] : cx<T>{0, 0};
+    __syncthreads();
     if (op == 0) {
-        const cx<T>* a = LU + (long long)k0 * lda + r;
-        for (int c = 0; c < w; ++c, a += lda) {
-            const cx<T> v = *a;
-            sr += v.re * sy[c].re - v.im * sy[c].im;
-            si += v.re * sy[c].im + v.im * sy[c].re;
+        const int row = t % B, grp = t / B, r = rb + row;
+        T ar = 0, ai = 0;
+        if (r < r1) {
+            const cx<T>* a = LU + (long long)(k0 + grp * CG) * lda + r;
+#pragma unroll 8
+            for (int cc = 0; cc < CG; ++cc) {
+                const int c = grp * CG + cc;
+                if (c < w) {
+                    const cx<T> v = a[(long long)cc * lda];
+                    ar += v.re * sy[c].re - v.im * sy[c].im;
+                    ai += v.re * sy[c].im + v.im * sy[c].re;
+                }
+            }
+        }
+        spr[grp][row] = ar;
+        spi[grp][row] = ai;
+        __syncthreads();
+        if (t < B && rb + t < r1) {
+            T re = 0, im = 0;
+#pragma unroll
+            for (int q = 0; q < P; ++q) { re += spr[q][t]; im += spi[q][t]; }
+            cx<T> v = vec[rb + t];
+            v.re -= re;
+            v.im -= im;
+            vec[rb + t] = v;
+            sres[t] = v;
         }
     } else {
-        const cx<T>* a = LU + (long long)r * lda + k0;
-        for (int c = 0; c < w; ++c) {
-            const cx<T> v = a[c];
-            sr += v.re * sy[c].re + v.im * sy[c].im;
-            si += v.re * sy[c].im - v.im * sy[c].re;
+        // the workgroup's rows of vec, staged once
+        if (t < B) sres[t] = rb + t < r1 ? vec[rb + t] : cx<T>{0, 0};
+        __syncthreads();
+        const int sub = t % L, rr = t / L;
+#pragma unroll 4
+        for (int pass = 0; pass < B / RP; ++pass) {
+            const int row = pass * RP + rr, r = rb + row;
+            T ar = 0, ai = 0;
+            if (r < r1) {
+                const cx<T>* a = LU + (long long)r * lda + k0 + sub * 4;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int c = sub * 4 + e;
+                    if (c < w) {
+                        const cx<T> v = a[e];
+                        ar += v.re * sy[c].re + v.im * sy[c].im;
+                        ai += v.re * sy[c].im - v.im * sy[c].re;
+                    }
+                }
+            }
+            for (int off = L / 2; off > 0; off >>= 1) {
+                ar += __shfl_xor(ar, off, L);
+                ai += __shfl_xor(ai, off, L);
+            }
+            if (sub == 0) {
+                sres[row].re -= ar;
+                sres[row].im -= ai;
+            }
         }
+        __syncthreads();
+        if (t < B && rb + t < r1) vec[rb + t] = sres[t];
     }
-    b[r].re -= sr;
-    b[r].im -= si;
+    if (rb != nxt0) return;
+    __syncthreads();
+    trsv_block_solve<T, B>(invn, sres, nw, spr, spi, out + nxt0);
+}
+
+// out[k0 + i] = M vec[k0 + i] for one diagonal block (the first of a sweep).
+template <typename T, int B>
+__global__ __launch_bounds__(256) void trsv_diag_kernel(int k0, int w, const cx<T>* __restrict__ inv, int op,
+                                                        const cx<T>* __restrict__ vec, cx<T>* __restrict__ out) {
+    __shared__ T spr[256 / B][B], spi[256 / B][B];
+    trsv_block_solve<T, B>(inv, vec + k0, w, spr, spi, out + k0);
 }
 
 // out[i] = in[perm[i]] (gather) or out[perm[i]] = in[i] (scatter)
@@ -1184,6 +1283,7 @@ static void getrf_block(int n, cx<T>* A, long long lda, int c0, int c1, int* ipi
 // per NB columns (the update is otherwise HBM-bound at K = NB).
 template <typename T>
 void getrf(int n, cx<T>* A, long long lda, int* ipiv, int* d_info, hipStream_t s, int variant) {
+    note_factorisation(A);
     SIE_HIP(hipMemsetAsync(d_info, 0, sizeof(int), s));
     for (int K0 = 0; K0 < n; K0 += kSP) {
         const int sp = std::min(kSP, n - K0);
@@ -1199,15 +1299,115 @@ void getrf(int n, cx<T>* A, long long lda, int* ipiv, int* d_info, hipStream_t s
     }
 }
 
-// Single right-hand-side solve by TB-blocked sweeps (op 0 = N, 2 = C).
+template <typename T>
+__global__ void copy_vec_kernel(int n, const cx<T>* __restrict__ in, cx<T>* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[i];
+}
+
+// Diagonal block size of the single-RHS sweeps: fewer, wider dependent steps
+// for FP32 (the sweeps are bound by the latency of their ~2n/B launches).
+template <typename T>
+struct SweepCfg {
+    static constexpr int B = std::is_same<T, float>::value ? 128 : 64;
+};
+
+// Generation of the last factorisation of each array, so that the sweeps can
+// keep the diagonal-block inverses of a factorisation across solves.
+static std::map<const void*, unsigned long long>& factor_generation() {
+    static std::map<const void*, unsigned long long> g;
+    return g;
+}
+static void note_factorisation(const void* A) {
+    static unsigned long long counter = 0;
+    factor_generation()[A] = ++counter;
+}
+
+// Single right-hand-side solve by B-blocked sweeps (op 0 = N, 2 = C): the
+// diagonal-block inverses (computed once per factorisation), then one launch
+// per block that updates the rows still to be solved and solves the next
+// diagonal block. The launch sequence is fixed by (op, n, factors, vector,
+// workspace), so it is captured once into a HIP graph and replayed:
+// iterative refinement calls it with the same pointers every sweep.
+// invL / invU: diagonal-block inverses for op N, their conjugate transposes for op C.
+template <typename T>
+static void record_sweeps(int op, int n, const cx<T>* LU, long long lda, cx<T>* b, const cx<T>* invL,
+                          const cx<T>* invU, cx<T>* y, cx<T>* t, const int* perm, hipStream_t s) {
+    constexpr int B = SweepCfg<T>::B;
+    const int nb = (n + B - 1) / B;
+    const unsigned g = (unsigned)((n + 255) / 256);
+    auto width = [&](int k) { return std::min(B, n - k); };
+    // forward sweep over blocks 0, B, ...: solve block k, update rows [k + B, n)
+    auto forward = [&](const cx<T>* inv, int sop, cx<T>* vec, cx<T>* out) {
+        trsv_diag_kernel<T, B><<<1, 256, 0, s>>>(0, width(0), inv, sop, vec, out);
+        for (int k = 0; k + B < n; k += B) {
+            const int rows = n - (k + B);
+            trsv_update_kernel<T, B><<<(unsigned)((rows + B - 1) / B), 256, 0, s>>>(
+                width(k), k, LU, lda, out, sop, vec, k + B, n, k + B, width(k + B),
+                inv + (size_t)((k + B) / B) * B * B, out);
+        }
+    };
+    // backward sweep over blocks (nb-1)B, ..., 0: solve block k, update rows [0, k)
+    auto backward = [&](const cx<T>* inv, int sop, cx<T>* vec, cx<T>* out) {
+        const int kl = (nb - 1) * B;
+        trsv_diag_kernel<T, B><<<1, 256, 0, s>>>(kl, width(kl), inv + (size_t)(nb - 1) * B * B, sop, vec, out);
+        for (int k = kl; k > 0; k -= B) {
+            trsv_update_kernel<T, B><<<(unsigned)(k / B), 256, 0, s>>>(
+                width(k), k, LU, lda, out, sop, vec, 0, k, k - B, B, inv + (size_t)((k - B) / B) * B * B, out);
+        }
+    };
+    if (op == 0) {
+        permute_kernel<T><<<g, 256, 0, s>>>(n, perm, b, t, 0);  // t = P^T b
+        forward(invL, 0, t, y);   // L y = t
+        backward(invU, 0, y, b);  // U b = y
+    } else {
+        copy_vec_kernel<T><<<g, 256, 0, s>>>(n, b, t);
+        forward(invU, 2, t, y);   // U^H y = b
+        backward(invL, 2, y, t);  // L^H t = y
+        permute_kernel<T><<<g, 256, 0, s>>>(n, perm, t, b, 1);  // b = P t
+    }
+    SIE_HIP(hipGetLastError());
+}
+
+struct SweepGraph {
+    int tsize, op, n;
+    const void* LU;
+    long long lda;
+    const void* b;
+    const void* w;
+    hipGraphExec_t exec;
+};
+
+struct SweepGraphCache {
+    std::vector<SweepGraph> entries;
+    hipStream_t capture = nullptr;
+    // diagonal-block inverses held in the workspace
+    const void* inv_LU = nullptr;
+    int inv_n = 0, inv_tsize = 0;
+    long long inv_lda = 0;
+    unsigned long long inv_gen = 0;
+    const void* inv_w = nullptr;
+    ~SweepGraphCache() {
+        for (auto& e : entries) (void)hipGraphExecDestroy(e.exec);
+        if (capture) (void)hipStreamDestroy(capture);
+    }
+};
+static SweepGraphCache& sweep_graphs() {
+    static SweepGraphCache c;
+    return c;
+}
+
 template <typename T>
 void getrs_vec(int op, int n, const cx<T>* LU, long long lda, const int* ipiv, cx<T>* b, hipStream_t s) {
-    const int nb = (n + TB - 1) / TB;
-    const size_t invsz = (size_t)nb * TB * TB * sizeof(cx<T>);
-    unsigned char* w = (unsigned char*)ws(5).get(2 * invsz + 2 * (size_t)n * sizeof(cx<T>) + (size_t)n * sizeof(int));
+    constexpr int B = SweepCfg<T>::B;
+    const int nb = (n + B - 1) / B;
+    const size_t invsz = (size_t)nb * B * B * sizeof(cx<T>);
+    unsigned char* w = (unsigned char*)ws(5).get(4 * invsz + 2 * (size_t)n * sizeof(cx<T>) + (size_t)n * sizeof(int));
     cx<T>* invL = (cx<T>*)w;
     cx<T>* invU = (cx<T>*)(w + invsz);
-    cx<T>* y = (cx<T>*)(w + 2 * invsz);
+    cx<T>* invLh = (cx<T>*)(w + 2 * invsz);
+    cx<T>* invUh = (cx<T>*)(w + 3 * invsz);
+    cx<T>* y = (cx<T>*)(w + 4 * invsz);
     cx<T>* t = y + n;
     int* perm = (int*)(t + n);
     // the row permutation of P^T, composed on the host from the swap sequence
@@ -1217,27 +1417,58 @@ void getrs_vec(int op, int n, const cx<T>* LU, long long lda, const int* ipiv, c
     for (int i = 0; i < n; ++i) hp[i] = i;
     for (int k = 0; k < n; ++k) std::swap(hp[k], hp[hpiv[k]]);
     SIE_HIP(hipMemcpyAsync(perm, hp.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
-    diag_inv_kernel<T><<<nb, TB, 0, s>>>(n, LU, lda, 0, invL);
-    diag_inv_kernel<T><<<nb, TB, 0, s>>>(n, LU, lda, 1, invU);
-    const unsigned g = (unsigned)((n + 255) / 256);
-    auto step = [&](int k, const cx<T>* inv, int sop, cx<T>* vec, cx<T>* out, int r0, int r1) {
-        const int wdt = std::min(TB, n - k);
-        const int rows = std::max(0, r1 - r0);
-        const unsigned blocks = (unsigned)std::max(1, (rows + 255) / 256);
-        trsv_step_kernel<T><<<blocks, 256, 0, s>>>(wdt, k, LU, lda, inv + (size_t)(k / TB) * TB * TB, sop, vec, out,
-                                                  r0, r1);
-    };
-    if (op == 0) {
-        permute_kernel<T><<<g, 256, 0, s>>>(n, perm, b, t, 0);  // t = P^T b
-        for (int k = 0; k < n; k += TB) step(k, invL, 0, t, y, k + TB, n);            // L y = t
-        for (int k = (nb - 1) * TB; k >= 0; k -= TB) step(k, invU, 0, y, b, 0, k);    // U b = y
-    } else {
-        SIE_HIP(hipMemcpyAsync(t, b, sizeof(cx<T>) * n, hipMemcpyDeviceToDevice, s));
-        for (int k = 0; k < n; k += TB) step(k, invU, 2, t, y, k + TB, n);            // U^H y = b
-        for (int k = (nb - 1) * TB; k >= 0; k -= TB) step(k, invL, 2, y, t, 0, k);    // L^H t = y
-        permute_kernel<T><<<g, 256, 0, s>>>(n, perm, t, b, 1);  // b = P t
+    SweepGraphCache& cache = sweep_graphs();
+    // inverses of the diagonal blocks: reused while the factors are those of
+    // the last factorisation of this array
+    const auto it = factor_generation().find(LU);
+    const unsigned long long gen = it == factor_generation().end() ? 0 : it->second;
+    if (!(gen != 0 && cache.inv_LU == LU && cache.inv_n == n && cache.inv_lda == lda &&
+          cache.inv_tsize == (int)sizeof(T) && cache.inv_gen == gen && cache.inv_w == w)) {
+        diag_inv_kernel<T, B><<<nb, B, 0, s>>>(n, LU, lda, 0, invL);
+        diag_inv_kernel<T, B><<<nb, B, 0, s>>>(n, LU, lda, 1, invU);
+        const long long tot = (long long)nb * B * B;
+        const unsigned gt = (unsigned)std::min<long long>((tot + 255) / 256, 1 << 16);
+        conj_transpose_blocks_kernel<T, B><<<gt, 256, 0, s>>>(tot, invL, invLh);
+        conj_transpose_blocks_kernel<T, B><<<gt, 256, 0, s>>>(tot, invU, invUh);
+        SIE_HIP(hipGetLastError());
+        cache.inv_LU = LU;
+        cache.inv_n = n;
+        cache.inv_lda = lda;
+        cache.inv_tsize = (int)sizeof(T);
+        cache.inv_gen = gen;
+        cache.inv_w = w;
     }
-    SIE_HIP(hipGetLastError());
+    const cx<T>* iL = op == 0 ? invL : invLh;
+    const cx<T>* iU = op == 0 ? invU : invUh;
+    if (nb < 8) {  // short sequences: launch directly
+        record_sweeps<T>(op, n, LU, lda, b, iL, iU, y, t, perm, s);
+        return;
+    }
+    hipGraphExec_t exec = nullptr;
+    for (auto& e : cache.entries)
+        if (e.tsize == (int)sizeof(T) && e.op == op && e.n == n && e.LU == LU && e.lda == lda && e.b == b && e.w == w)
+            exec = e.exec;
+    if (!exec) {
+        // drop graphs that point into an older workspace, keep at most 16
+        for (size_t i = 0; i < cache.entries.size();) {
+            if (cache.entries[i].w != w || cache.entries.size() >= 16) {
+                SIE_HIP(hipStreamSynchronize(s));
+                (void)hipGraphExecDestroy(cache.entries[i].exec);
+                cache.entries.erase(cache.entries.begin() + i);
+            } else {
+                ++i;
+            }
+        }
+        if (!cache.capture) SIE_HIP(hipStreamCreateWithFlags(&cache.capture, hipStreamNonBlocking));
+        hipGraph_t graph = nullptr;
+        SIE_HIP(hipStreamBeginCapture(cache.capture, hipStreamCaptureModeThreadLocal));
+        record_sweeps<T>(op, n, LU, lda, b, iL, iU, y, t, perm, cache.capture);
+        SIE_HIP(hipStreamEndCapture(cache.capture, &graph));
+        SIE_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        SIE_HIP(hipGraphDestroy(graph));
+        cache.entries.push_back(SweepGraph{(int)sizeof(T), op, n, LU, lda, b, w, exec});
+    }
+    SIE_HIP(hipGraphLaunch(exec, s));
 }
 
 // B = conj(B) on an n x nrhs block.

@@ -208,3 +208,36 @@ def test_gmres_edge_cases(sm):
     assert r.info == 1 and r.iterations == 2
     with pytest.raises(ValueError):
         sm.linalg.gmres(Ad, torch.from_numpy(b).cuda(), m=0)
+
+
+@pytest.mark.parametrize("op", ["N", "C"])
+def test_single_rhs_graph_replays(sm, op):
+    # single-RHS solves replay a captured launch sequence: same factors and
+    # vector buffer, new contents each time, eager work in between
+    rng = np.random.default_rng(41)
+    n = 900
+    A = crandn(rng, n, n)
+    lu = sm.linalg.DenseLuSolver()
+    Ad = to_dev(A, torch.complex64)
+    lu.factorize(Ad)
+    buf = torch.empty(n, dtype=torch.complex64, device="cuda")
+    M = A if op == "N" else A.conj().T
+    for rep in range(4):
+        b = crandn(rng, n)
+        buf.copy_(torch.from_numpy(b).to(torch.complex64))
+        noise = torch.randn(256, 256, device="cuda") @ torch.randn(256, 256, device="cuda")  # eager work
+        lu.solve(buf, op=op)
+        x = buf.cpu().numpy().astype(np.complex128)
+        res = np.linalg.norm(M @ x - b) / (np.linalg.norm(M) * np.linalg.norm(x))
+        assert res < 5e-6, (rep, res)
+        del noise
+    # a second factorisation in the same buffers: the replay reads the new factors
+    A2 = crandn(rng, n, n)
+    Ad.copy_(to_dev(A2, torch.complex64))
+    lu.factorize(Ad)
+    b = crandn(rng, n)
+    buf.copy_(torch.from_numpy(b).to(torch.complex64))
+    lu.solve(buf, op=op)
+    M2 = A2 if op == "N" else A2.conj().T
+    x = buf.cpu().numpy().astype(np.complex128)
+    assert np.linalg.norm(M2 @ x - b) / (np.linalg.norm(M2) * np.linalg.norm(x)) < 5e-6
