@@ -690,53 +690,72 @@ __global__ void laswp_kernel(cx<T>* __restrict__ A, long long lda, int c0, int n
 }
 
 // ============================================== small triangular solves
-// In-place solve of a W x W triangular system on ncols right-hand columns:
+// In-place solve of a w x w (w <= WM) triangular system on ncols right-hand columns:
 //   uplo 0: unit lower L (forward), 1: non-unit upper U (backward),
 //   2: conj-transposed non-unit upper U^H (forward), 3: conj-transposed unit lower L^H (backward).
-// The triangle T lives at A (column-major, lda); B at B (ldb). One lane per column.
-template <typename T>
-__global__ void trsm_small_kernel(int w, const cx<T>* __restrict__ Tm, long long lda, cx<T>* __restrict__ B,
-                                  long long ldb, int ncols, int uplo) {
-    extern __shared__ unsigned char smem_raw[];
-    cx<T>* sT = reinterpret_cast<cx<T>*>(smem_raw);  // w x w, column-major
-    for (int t = threadIdx.x; t < w * w; t += blockDim.x) sT[t] = Tm[(long long)(t / w) * lda + (t % w)];
+// The triangle T lives at A (column-major, lda); B at B (ldb). One lane per
+// column; the column is held in registers (fully unrolled, static indices).
+template <typename T, int WM>
+__global__ __launch_bounds__(256) void trsm_small_kernel(int w, const cx<T>* __restrict__ Tm, long long lda,
+                                                         cx<T>* __restrict__ B, long long ldb, int ncols, int uplo) {
+    __shared__ cx<T> sT[WM * WM];  // column-major w x w
+    for (int t = threadIdx.x; t < w * w; t += blockDim.x) sT[(t / w) * WM + (t % w)] = Tm[(long long)(t / w) * lda + (t % w)];
     __syncthreads();
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncols) return;
     cx<T>* b = B + (long long)c * ldb;
-    auto at = [&](int i, int j) { return sT[j * w + i]; };
-    auto mul = [](cx<T> a, cx<T> x) { return cx<T>{a.re * x.re - a.im * x.im, a.re * x.im + a.im * x.re}; };
-    auto cmulc = [](cx<T> a, cx<T> x) { return cx<T>{a.re * x.re + a.im * x.im, a.re * x.im - a.im * x.re}; };  // conj(a) x
-    auto divd = [](cx<T> x, cx<T> d) {
+    auto at = [&](int i, int j) { return sT[j * WM + i]; };
+    cx<T> x[WM];
+#pragma unroll
+    for (int i = 0; i < WM; ++i) x[i] = i < w ? b[i] : cx<T>{0, 0};
+    auto msub = [](cx<T>& s, cx<T> a, cx<T> v) {  // s -= a v
+        s.re -= a.re * v.re - a.im * v.im;
+        s.im -= a.re * v.im + a.im * v.re;
+    };
+    auto msubc = [](cx<T>& s, cx<T> a, cx<T> v) {  // s -= conj(a) v
+        s.re -= a.re * v.re + a.im * v.im;
+        s.im -= a.re * v.im - a.im * v.re;
+    };
+    auto divd = [](cx<T> v, cx<T> d) {
         const T q = d.re * d.re + d.im * d.im;
-        return cx<T>{(x.re * d.re + x.im * d.im) / q, (x.im * d.re - x.re * d.im) / q};
+        return cx<T>{(v.re * d.re + v.im * d.im) / q, (v.im * d.re - v.re * d.im) / q};
     };
     if (uplo == 0) {
-        for (int i = 0; i < w; ++i) {
-            cx<T> s = b[i];
-            for (int k = 0; k < i; ++k) { const cx<T> m = mul(at(i, k), b[k]); s.re -= m.re; s.im -= m.im; }
-            b[i] = s;
-        }
+#pragma unroll
+        for (int i = 1; i < WM; ++i)
+#pragma unroll
+            for (int k = 0; k < i; ++k)
+                if (i < w) msub(x[i], at(i, k), x[k]);
     } else if (uplo == 1) {
-        for (int i = w - 1; i >= 0; --i) {
-            cx<T> s = b[i];
-            for (int k = i + 1; k < w; ++k) { const cx<T> m = mul(at(i, k), b[k]); s.re -= m.re; s.im -= m.im; }
-            b[i] = divd(s, at(i, i));
+#pragma unroll
+        for (int i = WM - 1; i >= 0; --i) {
+            if (i < w) {
+#pragma unroll
+                for (int k = i + 1; k < WM; ++k)
+                    if (k < w) msub(x[i], at(i, k), x[k]);
+                x[i] = divd(x[i], at(i, i));
+            }
         }
     } else if (uplo == 2) {  // U^H lower: (U^H)(i,k) = conj(U(k,i))
-        for (int i = 0; i < w; ++i) {
-            cx<T> s = b[i];
-            for (int k = 0; k < i; ++k) { const cx<T> m = cmulc(at(k, i), b[k]); s.re -= m.re; s.im -= m.im; }
-            const cx<T> d = {at(i, i).re, -at(i, i).im};
-            b[i] = divd(s, d);
+#pragma unroll
+        for (int i = 0; i < WM; ++i) {
+            if (i < w) {
+#pragma unroll
+                for (int k = 0; k < i; ++k) msubc(x[i], at(k, i), x[k]);
+                const cx<T> d = {at(i, i).re, -at(i, i).im};
+                x[i] = divd(x[i], d);
+            }
         }
     } else {  // L^H upper unit: (L^H)(i,k) = conj(L(k,i))
-        for (int i = w - 1; i >= 0; --i) {
-            cx<T> s = b[i];
-            for (int k = i + 1; k < w; ++k) { const cx<T> m = cmulc(at(k, i), b[k]); s.re -= m.re; s.im -= m.im; }
-            b[i] = s;
-        }
+#pragma unroll
+        for (int i = WM - 1; i >= 0; --i)
+#pragma unroll
+            for (int k = i + 1; k < WM; ++k)
+                if (i < w && k < w) msubc(x[i], at(k, i), x[k]);
     }
+#pragma unroll
+    for (int i = 0; i < WM; ++i)
+        if (i < w) b[i] = x[i];
 }
 
 // ================================================================= GEMV
@@ -1142,8 +1161,9 @@ template <typename T>
 static void trsm_small(int w, const cx<T>* Tm, long long lda, cx<T>* B, long long ldb, int ncols, int uplo,
                        hipStream_t s) {
     if (ncols <= 0 || w <= 0) return;
-    const size_t sh = (size_t)w * w * sizeof(cx<T>);
-    trsm_small_kernel<T><<<(ncols + 255) / 256, 256, sh, s>>>(w, Tm, lda, B, ldb, ncols, uplo);
+    constexpr int WM = 32;  // panel width (FP32) and the multi-RHS solve block
+    require(w <= WM, "internal: triangular block wider than the register tile");
+    trsm_small_kernel<T, WM><<<(ncols + 255) / 256, 256, 0, s>>>(w, Tm, lda, B, ldb, ncols, uplo);
     SIE_HIP(hipGetLastError());
 }
 
