@@ -82,6 +82,9 @@ int sie_calderon_pair_blocks(sie_mesh_t mesh, const sie_calderon_params* p,
 
 #define SIE_ASM_CALDERON 1
 #define SIE_ASM_MASS 2
+#define SIE_ASM_UNSCALED 4 /* eta0 -> 1: assemble_Z_calderon_kernel (sie_kernels_assembly.cuh:24-148)
+                              and assemble_Z_mass_kernel (:282-452), the operator of the
+                              reference's matrix-free matvec */
 /* eta0-scaled SIE-GSTC matrix D A D (dim 4E), overwriting Z_dev.
  * Replaces assemble_system_matrix (gstc_sie.cu:244-317): the Calderon kernel
  * assemble_Z_calderon_eta0_scaled_kernel (sie_kernels_assembly.cuh:156-272)
@@ -102,6 +105,21 @@ int sie_assemble_pec(sie_mesh_t mesh, const sie_calderon_params* p, double* Z_de
  * src/sie_mom/solvers/dielectric/dielectric_kernels.cu:23-150). */
 int sie_assemble_pmchwt(sie_mesh_t mesh, const sie_calderon_params* p, double* Z_dev,
                         long long ldz, void* stream);
+
+/* Matrix-free products y = Z x with the operators above, Z never stored:
+ * the blocks of each chunk of source faces are computed and applied at once
+ * (x and y: device vectors of the operator's dimension, distinct).
+ * sie_gstc_matvec replaces matvec_Z_calderon_kernel + matvec_Z_mass_kernel
+ * (sie_kernels_matvec.cuh:9-327, launched by PhysicsMatVec,
+ * test_gmres_solver.cu:247-287); pass SIE_ASM_UNSCALED for that operator.
+ * Synchronises the stream. */
+int sie_gstc_matvec(sie_mesh_t mesh, const sie_calderon_params* p, const double* tensors_dev,
+                    int uniform, int basis, int flags, const double* x_dev, double* y_dev,
+                    void* stream);
+int sie_pec_matvec(sie_mesh_t mesh, const sie_calderon_params* p, const double* x_dev,
+                   double* y_dev, void* stream);
+int sie_pmchwt_matvec(sie_mesh_t mesh, const sie_calderon_params* p, const double* x_dev,
+                      double* y_dev, void* stream);
 
 /* Plane wave bound to its medium (cuda::PlaneWave, include/sie_mom/excitation/
  * source_functors.cuh:14-42; region 1 interior, 2 exterior). */

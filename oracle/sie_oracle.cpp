@@ -618,15 +618,18 @@ void oracle_pair_blocks_all(const double* V, const int* F, int nf, int rule, flo
 
 // Full eta0-scaled GSTC matrix (sie_kernels_assembly.cuh:156-272 Calderon part,
 // :460-594 mass part; gstc_sie.cu:244-317 launch parameters). Column-major.
-void oracle_assemble_gstc(const double* V, const int* F, const int* FE, int nf, int ne, int rule,
-                          float k1, float k2, float mu_r1, float mu_r2, float eps_r1, float eps_r2,
-                          float mu0, float eps0, float omega, float thr, const double* tensors,
-                          int uniform, int basis, int with_calderon, int with_mass, double* Z) {
+// scaled = 0: the unscaled operator of assemble_Z_calderon_kernel (:24-148:
+// jw mu TL, TK, -TK, jw eps TL) and assemble_Z_mass_kernel (:282-452: no eta0
+// factors), i.e. the same expressions with eta0 = 1.
+void oracle_assemble_gstc_ex(const double* V, const int* F, const int* FE, int nf, int ne, int rule,
+                             float k1, float k2, float mu_r1, float mu_r2, float eps_r1, float eps_r2,
+                             float mu0, float eps0, float omega, float thr, const double* tensors,
+                             int uniform, int basis, int with_calderon, int with_mass, int scaled, double* Z) {
     Rule q = rule_of(rule);
     const size_t E = ne, dim = 4 * E;
     std::memset(Z, 0, sizeof(double) * 2 * dim * dim);
     // make_eta0_scaled_calderon_factors (sie_scaling.cuh:48-61)
-    const double eta0 = std::sqrt((double)(mu0 / eps0));
+    const double eta0 = scaled ? std::sqrt((double)(mu0 / eps0)) : 1.0;
     const double mu1 = (double)(mu_r1 * mu0), mu2 = (double)(mu_r2 * mu0);
     const double ep1 = (double)(eps_r1 * eps0), ep2 = (double)(eps_r2 * eps0);
     const C jj1 = cscl({0.0, (double)omega}, mu1), jj2 = cscl({0.0, (double)omega}, mu2);
@@ -669,7 +672,7 @@ void oracle_assemble_gstc(const double* V, const int* F, const int* FE, int nf, 
         }
     }
     if (!with_mass) return;
-    const float e0f = std::sqrt(mu0 / eps0);
+    const float e0f = scaled ? std::sqrt(mu0 / eps0) : 1.f;
     const float e0sq = e0f * e0f;
     for (int m = 0; m < nf; ++m) {
         V3 t[3]; int idx[3];
@@ -715,6 +718,14 @@ void oracle_assemble_gstc(const double* V, const int* F, const int* FE, int nf, 
             }
         }
     }
+}
+
+void oracle_assemble_gstc(const double* V, const int* F, const int* FE, int nf, int ne, int rule,
+                          float k1, float k2, float mu_r1, float mu_r2, float eps_r1, float eps_r2,
+                          float mu0, float eps0, float omega, float thr, const double* tensors,
+                          int uniform, int basis, int with_calderon, int with_mass, double* Z) {
+    oracle_assemble_gstc_ex(V, F, FE, nf, ne, rule, k1, k2, mu_r1, mu_r2, eps_r1, eps_r2, mu0, eps0, omega, thr,
+                            tensors, uniform, basis, with_calderon, with_mass, 1, Z);
 }
 
 // PEC T-EFIE (sie_kernels_assembly.cuh:610-705; pec_solver.cu:352-359 passes k2 twice).

@@ -471,7 +471,9 @@ __global__ __launch_bounds__(256) void gather_kernel(
     int nf, int ne, int n0, int nc, const int* __restrict__ elist, int nlist,
     const int* __restrict__ ef, const int* __restrict__ el, const cplx* __restrict__ W,
     const cplx* __restrict__ M /* nf x 144, GSTC only */, Epi ep, cplx* __restrict__ Z,
-    long long ldz) {
+    long long ldz, int col_stride) {
+    // col_stride 0: columns of the full matrix (edge id + block E); > 0: a slab
+    // whose block c holds the chunk's edges in list order at c * col_stride
     constexpr int NS = NK * 18;
     const int em = blockIdx.x * blockDim.x + threadIdx.x;
     const int li = uniform(blockIdx.y);
@@ -499,6 +501,7 @@ __global__ __launch_bounds__(256) void gather_kernel(
         }
     }
     const long long E = ne;
+    auto col = [&](int c) -> long long { return col_stride > 0 ? (long long)li + (long long)c * col_stride : en + c * E; };
     if (KIND == K_GSTC) {
         const cplx TL1 = acc[0], TK1 = acc[1];
         const cplx TL2 = NK == 2 ? acc[2] : acc[0], TK2 = NK == 2 ? acc[3] : acc[1];
@@ -545,10 +548,10 @@ __global__ __launch_bounds__(256) void gather_kernel(
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
                 const bool cal = (r >> 1) == (c >> 1);
-                if (cal || shared_g >= 0) zacc(Z, ldz, em + r * E, en + c * E, blk[r][c]);
+                if (cal || shared_g >= 0) zacc(Z, ldz, em + r * E, col(c), blk[r][c]);
             }
     } else if (KIND == K_PEC) {
-        zacc(Z, ldz, em, en, cmul(ep.c[0], NK == 2 ? acc[2] : acc[0]));
+        zacc(Z, ldz, em, col(0), cmul(ep.c[0], NK == 2 ? acc[2] : acc[0]));
     } else if (KIND == K_PMCHWT) {
         const cplx TL1 = acc[0], TK1 = acc[1];
         const cplx TL2 = NK == 2 ? acc[2] : acc[0], TK2 = NK == 2 ? acc[3] : acc[1];
@@ -558,10 +561,10 @@ __global__ __launch_bounds__(256) void gather_kernel(
         const cplx d = cmul(ep.c[2], TL1), e = cmul(ep.c[3], TL2);
         const cplx zmm = {d.re + e.re, d.im + e.im};
         const double et = ep.eta, et2 = ep.jm;  // eta0 and (float) eta0^2
-        zacc(Z, ldz, em, en, zjj);
-        zacc(Z, ldz, em, en + E, {tk.re * et, tk.im * et});
-        zacc(Z, ldz, em + E, en, {-tk.re * et, -tk.im * et});
-        zacc(Z, ldz, em + E, en + E, {zmm.re * et2, zmm.im * et2});
+        zacc(Z, ldz, em, col(0), zjj);
+        zacc(Z, ldz, em, col(1), {tk.re * et, tk.im * et});
+        zacc(Z, ldz, em + E, col(0), {-tk.re * et, -tk.im * et});
+        zacc(Z, ldz, em + E, col(1), {zmm.re * et2, zmm.im * et2});
     }
 }
 
@@ -592,7 +595,7 @@ __global__ __launch_bounds__(64) void mass_kernel(
     int nf, int Q, const double* __restrict__ V, const int* __restrict__ F,
     const double* __restrict__ len, const double* __restrict__ area, const float* __restrict__ qxy,
     const float* __restrict__ qw, const cplx* __restrict__ tensors, int uniform_t, int basis,
-    float mu0, float eps0, cplx* __restrict__ M) {
+    float mu0, float eps0, int scaled, cplx* __restrict__ M) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= nf) return;
     V3 t[3];
@@ -705,7 +708,8 @@ __global__ __launch_bounds__(64) void mass_kernel(
     }
     const float ar = (float)area[f];
     const float sc = 1.f / (4.f * ar);
-    const float e0 = sqrtf(mu0 / eps0);
+    // unscaled operator (assemble_Z_mass_kernel, sie_kernels_assembly.cuh:282-452): eta0 -> 1
+    const float e0 = scaled ? sqrtf(mu0 / eps0) : 1.f;
     const float e0sq = e0 * e0;
     // row/col scaling D = diag(1, eta0, 1, eta0)
     const double rs[4] = {1.0, (double)e0, 1.0, (double)e0};
@@ -735,6 +739,15 @@ __global__ __launch_bounds__(64) void mass_kernel(
 
 // Grid-stride fill: the dispatch grid (blocks x threads) must stay below 2^32
 // work-items, so large matrices are covered by a capped grid.
+// xs[c L + li] = x[c E + elist[li]] for the slab matvec.
+__global__ void slab_x_kernel(int L, int blocks, long long E, const int* __restrict__ elist,
+                              const cplx* __restrict__ x, cplx* __restrict__ xs) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= L * blocks) return;
+    const int c = t / L, li = t - c * L;
+    xs[t] = x[c * E + elist[li]];
+}
+
 __global__ void fill_kernel(cplx* __restrict__ Z, long long ldz, int rows, int cols) {
     const long long tot = (long long)rows * cols;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < tot;
@@ -834,6 +847,7 @@ struct Assembler {
     DevBuf<int2> near;
     DevBuf<unsigned int> count;
     DevBuf<int> elist;
+    DevBuf<cplx> slab, xs;  // matrix-free products: the column slab of a chunk and its x entries
     unsigned int* h_count = nullptr;
     size_t scratch_bytes = size_t(4) << 30;  // W chunk budget
 
@@ -926,18 +940,24 @@ static int NKof(const sie_calderon_params* p) { return (p->k1 == p->k2) ? 1 : 2;
 
 template <int NK, int KIND>
 static void assemble_generic(sie_mesh* h, const sie_calderon_params* p, const Epi& ep, const cplx* Mblk,
-                             cplx* Z, long long ldz, int dim, hipStream_t s) {
+                             cplx* Z, long long ldz, int dim, hipStream_t s, const cplx* x = nullptr,
+                             cplx* y = nullptr) {
+    // x != nullptr: y = Z x without storing Z. Each chunk of source faces is
+    // gathered into a slab of the columns of its edges, then y += slab x(those edges).
     Mesh& m = h->m;
     Assembler& A = *h->as;
     A.prepare(p->rule, s);
     const int nc_max = A.chunk_faces(NK);
     const size_t wsz = (size_t)nc_max * NK * 18 * m.nf;
     if (A.W.n < wsz) A.W.alloc(wsz);
-    {
+    const int blocks = KIND == K_GSTC ? 4 : (KIND == K_PMCHWT ? 2 : 1);
+    if (x) {
+        fill_kernel<<<(unsigned)std::min<long long>((dim + 255) / 256, 1 << 16), 256, 0, s>>>(y, dim, dim, 1);
+    } else {
         const long long tot = (long long)dim * dim;
         fill_kernel<<<(unsigned)std::min<long long>((tot + 255) / 256, 1 << 16), 256, 0, s>>>(Z, ldz, dim, dim);
-        SIE_HIP(hipGetLastError());
     }
+    SIE_HIP(hipGetLastError());
     std::vector<int> list;
     list.reserve(m.ne);
     for (int n0 = 0; n0 < m.nf; n0 += nc_max) {
@@ -949,11 +969,28 @@ static void assemble_generic(sie_mesh* h, const sie_calderon_params* p, const Ep
             const int g0 = m.ef[0][e], g1 = m.ef[1][e];
             if ((g0 >= n0 && g0 < n0 + nc) || (g1 >= n0 && g1 < n0 + nc)) list.push_back(e);
         }
+        const int L = (int)list.size();
         if (A.elist.n < list.size()) A.elist.alloc(m.ne);
         SIE_HIP(hipMemcpyAsync(A.elist.p, list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice, s));
-        dim3 grid((m.ne + 255) / 256, (unsigned)list.size());
-        gather_kernel<NK, KIND><<<grid, 256, 0, s>>>(m.nf, m.ne, n0, nc, A.elist.p, (int)list.size(), m.d_ef.p,
-                                                     m.d_el.p, A.W.p, Mblk, ep, Z, ldz);
+        dim3 grid((m.ne + 255) / 256, (unsigned)L);
+        if (x) {
+            const size_t cols = (size_t)blocks * L;
+            if (A.slab.n < (size_t)dim * cols) A.slab.alloc((size_t)dim * cols);
+            if (A.xs.n < cols) A.xs.alloc((size_t)blocks * m.ne);
+            const long long tot = (long long)dim * (long long)cols;
+            fill_kernel<<<(unsigned)std::min<long long>((tot + 255) / 256, 1 << 16), 256, 0, s>>>(A.slab.p, dim, dim,
+                                                                                                  (int)cols);
+            gather_kernel<NK, KIND><<<grid, 256, 0, s>>>(m.nf, m.ne, n0, nc, A.elist.p, L, m.d_ef.p, m.d_el.p, A.W.p,
+                                                         Mblk, ep, A.slab.p, dim, L);
+            slab_x_kernel<<<(unsigned)((cols + 255) / 256), 256, 0, s>>>(L, blocks, m.ne, A.elist.p, x, A.xs.p);
+            SIE_HIP(hipGetLastError());
+            const int rc = sie_zgemv(0, dim, (int)cols, 1.0, (const double*)A.slab.p, dim, (const double*)A.xs.p,
+                                     1.0, (double*)y, s);
+            if (rc != SIE_OK) throw Error(rc, sie_last_error());
+        } else {
+            gather_kernel<NK, KIND><<<grid, 256, 0, s>>>(m.nf, m.ne, n0, nc, A.elist.p, L, m.d_ef.p, m.d_el.p, A.W.p,
+                                                         Mblk, ep, Z, ldz, 0);
+        }
         SIE_HIP(hipGetLastError());
         SIE_HIP(hipStreamSynchronize(s));  // list buffer reused by the next chunk
     }
@@ -1046,17 +1083,18 @@ int sie_calderon_pair_blocks(sie_mesh_t mesh, const sie_calderon_params* p, doub
     SIE_GUARD_END
 }
 
-int sie_assemble_gstc(sie_mesh_t mesh, const sie_calderon_params* p, const double* tensors_dev, int uniform_t,
-                      int basis, int flags, double* Z_dev, long long ldz, void* stream) {
-    SIE_GUARD_BEGIN
+}  // extern "C"
+
+// The three operators, assembled into Z (x == nullptr) or applied as y = Z x.
+static void run_gstc(sie_mesh_t mesh, const sie_calderon_params* p, const double* tensors_dev, int uniform_t,
+                     int basis, int flags, cplx* Z, long long ldz, const cplx* x, cplx* y, hipStream_t s) {
     check_params(mesh, p);
-    require(Z_dev != nullptr, "null matrix");
     require(basis == 0 || basis == 1, "unsupported GSTC basis type");
     const int E = mesh->m.ne;
     const long long dim = 4LL * E;
     require(dim <= 2147483647LL, "SIE matrix dimension exceeds the int limit");
-    require(ldz >= dim, "leading dimension smaller than the matrix dimension");
-    hipStream_t s = (hipStream_t)stream;
+    if (!x) require(ldz >= dim, "leading dimension smaller than the matrix dimension");
+    const bool unscaled = (flags & SIE_ASM_UNSCALED) != 0;
     DevBuf<cplx> M;
     if (flags & SIE_ASM_MASS) {
         require(tensors_dev != nullptr, "GSTC tensors are required for the mass blocks");
@@ -1066,12 +1104,13 @@ int sie_assemble_gstc(sie_mesh_t mesh, const sie_calderon_params* p, const doubl
         mass_kernel<<<(mesh->m.nf + 63) / 64, 64, 0, s>>>(mesh->m.nf, A.quad.n, mesh->m.d_V.p, mesh->m.d_F.p,
                                                           mesh->m.d_len.p, mesh->area.p, A.quad.xy.p, A.quad.w.p,
                                                           (const cplx*)tensors_dev, uniform_t, basis, p->mu0,
-                                                          p->eps0, M.p);
+                                                          p->eps0, unscaled ? 0 : 1, M.p);
         SIE_HIP(hipGetLastError());
     }
     Epi ep{};
-    // make_eta0_scaled_calderon_factors: eta0 = sqrt(double(float(mu0/eps0)))
-    const double eta0 = std::sqrt((double)(p->mu0 / p->eps0));
+    // make_eta0_scaled_calderon_factors: eta0 = sqrt(double(float(mu0/eps0)));
+    // the unscaled operator (assemble_Z_calderon_kernel, sie_kernels_assembly.cuh:24-148) has eta0 = 1
+    const double eta0 = unscaled ? 1.0 : std::sqrt((double)(p->mu0 / p->eps0));
     const double om = p->omega;
     const double mu[2] = {(double)(p->mu_r1 * p->mu0), (double)(p->mu_r2 * p->mu0)};
     const double ep_[2] = {(double)(p->eps_r1 * p->eps0), (double)(p->eps_r2 * p->eps0)};
@@ -1085,35 +1124,30 @@ int sie_assemble_gstc(sie_mesh_t mesh, const sie_calderon_params* p, const doubl
         ep.jm = 0;
     }
     if (NKof(p) == 1)
-        assemble_generic<1, K_GSTC>(mesh, p, ep, M.p, (cplx*)Z_dev, ldz, (int)dim, s);
+        assemble_generic<1, K_GSTC>(mesh, p, ep, M.p, Z, ldz, (int)dim, s, x, y);
     else
-        assemble_generic<2, K_GSTC>(mesh, p, ep, M.p, (cplx*)Z_dev, ldz, (int)dim, s);
+        assemble_generic<2, K_GSTC>(mesh, p, ep, M.p, Z, ldz, (int)dim, s, x, y);
     SIE_HIP(hipStreamSynchronize(s));
-    SIE_GUARD_END
 }
 
-int sie_assemble_pec(sie_mesh_t mesh, const sie_calderon_params* p, double* Z_dev, long long ldz, void* stream) {
-    SIE_GUARD_BEGIN
+static void run_pec(sie_mesh_t mesh, const sie_calderon_params* p, cplx* Z, long long ldz, const cplx* x, cplx* y,
+                    hipStream_t s) {
     check_params(mesh, p);
-    require(Z_dev != nullptr, "null matrix");
     const int E = mesh->m.ne;
-    require(ldz >= E, "leading dimension smaller than the matrix dimension");
+    if (!x) require(ldz >= E, "leading dimension smaller than the matrix dimension");
     Epi ep{};
     ep.c[0] = {0.0, (double)(p->omega * p->mu_r2 * p->mu0)};
     sie_calderon_params q = *p;
     q.k1 = p->k2;  // the reference passes k2 for both wavenumbers
-    assemble_generic<1, K_PEC>(mesh, &q, ep, nullptr, (cplx*)Z_dev, ldz, E, (hipStream_t)stream);
-    SIE_HIP(hipStreamSynchronize((hipStream_t)stream));
-    SIE_GUARD_END
+    assemble_generic<1, K_PEC>(mesh, &q, ep, nullptr, Z, ldz, E, s, x, y);
+    SIE_HIP(hipStreamSynchronize(s));
 }
 
-int sie_assemble_pmchwt(sie_mesh_t mesh, const sie_calderon_params* p, double* Z_dev, long long ldz,
-                        void* stream) {
-    SIE_GUARD_BEGIN
+static void run_pmchwt(sie_mesh_t mesh, const sie_calderon_params* p, cplx* Z, long long ldz, const cplx* x,
+                       cplx* y, hipStream_t s) {
     check_params(mesh, p);
-    require(Z_dev != nullptr, "null matrix");
     const long long dim = 2LL * mesh->m.ne;
-    require(ldz >= dim, "leading dimension smaller than the matrix dimension");
+    if (!x) require(ldz >= dim, "leading dimension smaller than the matrix dimension");
     Epi ep{};
     const float m1 = p->mu_r1 * p->mu0, m2 = p->mu_r2 * p->mu0, e1 = p->eps_r1 * p->eps0, e2 = p->eps_r2 * p->eps0;
     const double om = p->omega;
@@ -1125,10 +1159,60 @@ int sie_assemble_pmchwt(sie_mesh_t mesh, const sie_calderon_params* p, double* Z
     ep.eta = eta0;
     ep.jm = (double)(eta0 * eta0);
     if (NKof(p) == 1)
-        assemble_generic<1, K_PMCHWT>(mesh, p, ep, nullptr, (cplx*)Z_dev, ldz, (int)dim, (hipStream_t)stream);
+        assemble_generic<1, K_PMCHWT>(mesh, p, ep, nullptr, Z, ldz, (int)dim, s, x, y);
     else
-        assemble_generic<2, K_PMCHWT>(mesh, p, ep, nullptr, (cplx*)Z_dev, ldz, (int)dim, (hipStream_t)stream);
-    SIE_HIP(hipStreamSynchronize((hipStream_t)stream));
+        assemble_generic<2, K_PMCHWT>(mesh, p, ep, nullptr, Z, ldz, (int)dim, s, x, y);
+    SIE_HIP(hipStreamSynchronize(s));
+}
+
+extern "C" {
+
+int sie_assemble_gstc(sie_mesh_t mesh, const sie_calderon_params* p, const double* tensors_dev, int uniform_t,
+                      int basis, int flags, double* Z_dev, long long ldz, void* stream) {
+    SIE_GUARD_BEGIN
+    require(Z_dev != nullptr, "null matrix");
+    run_gstc(mesh, p, tensors_dev, uniform_t, basis, flags, (cplx*)Z_dev, ldz, nullptr, nullptr,
+             (hipStream_t)stream);
+    SIE_GUARD_END
+}
+
+int sie_assemble_pec(sie_mesh_t mesh, const sie_calderon_params* p, double* Z_dev, long long ldz, void* stream) {
+    SIE_GUARD_BEGIN
+    require(Z_dev != nullptr, "null matrix");
+    run_pec(mesh, p, (cplx*)Z_dev, ldz, nullptr, nullptr, (hipStream_t)stream);
+    SIE_GUARD_END
+}
+
+int sie_assemble_pmchwt(sie_mesh_t mesh, const sie_calderon_params* p, double* Z_dev, long long ldz,
+                        void* stream) {
+    SIE_GUARD_BEGIN
+    require(Z_dev != nullptr, "null matrix");
+    run_pmchwt(mesh, p, (cplx*)Z_dev, ldz, nullptr, nullptr, (hipStream_t)stream);
+    SIE_GUARD_END
+}
+
+int sie_gstc_matvec(sie_mesh_t mesh, const sie_calderon_params* p, const double* tensors_dev, int uniform_t,
+                    int basis, int flags, const double* x_dev, double* y_dev, void* stream) {
+    SIE_GUARD_BEGIN
+    require(x_dev != nullptr && y_dev != nullptr && x_dev != y_dev, "matvec needs distinct x and y vectors");
+    run_gstc(mesh, p, tensors_dev, uniform_t, basis, flags, nullptr, 0, (const cplx*)x_dev, (cplx*)y_dev,
+             (hipStream_t)stream);
+    SIE_GUARD_END
+}
+
+int sie_pec_matvec(sie_mesh_t mesh, const sie_calderon_params* p, const double* x_dev, double* y_dev,
+                   void* stream) {
+    SIE_GUARD_BEGIN
+    require(x_dev != nullptr && y_dev != nullptr && x_dev != y_dev, "matvec needs distinct x and y vectors");
+    run_pec(mesh, p, nullptr, 0, (const cplx*)x_dev, (cplx*)y_dev, (hipStream_t)stream);
+    SIE_GUARD_END
+}
+
+int sie_pmchwt_matvec(sie_mesh_t mesh, const sie_calderon_params* p, const double* x_dev, double* y_dev,
+                      void* stream) {
+    SIE_GUARD_BEGIN
+    require(x_dev != nullptr && y_dev != nullptr && x_dev != y_dev, "matvec needs distinct x and y vectors");
+    run_pmchwt(mesh, p, nullptr, 0, (const cplx*)x_dev, (cplx*)y_dev, (hipStream_t)stream);
     SIE_GUARD_END
 }
 

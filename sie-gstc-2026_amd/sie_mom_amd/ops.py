@@ -44,23 +44,73 @@ def calderon_pair_blocks(mesh, params):
     return out
 
 
-def assemble_gstc(mesh, options: GstcOptions, tensors, calderon=True, mass=True, out=None):
-    """eta0-scaled SIE-GSTC matrix (dim 4E). tensors: complex128 device tensor of
-    32 values per face (or 32 if options.tensors_are_uniform)."""
-    dim = 4 * mesh.ne
-    p = calderon_params(options)
+def _gstc_args(mesh, options, tensors, calderon, mass, scaled):
     nrec = 1 if options.tensors_are_uniform else mesh.nf
     if tensors.numel() != 32 * nrec:
         raise ValueError(f"GSTC tensor count mismatch: expected {32 * nrec}, got {tensors.numel()}")
     t = tensors.to(device="cuda", dtype=torch.complex128).contiguous()
+    flags = (1 if calderon else 0) | (2 if mass else 0) | (0 if scaled else 4)
+    basis = {"face": 0, "spherical": 1}[options.basis]
+    return t, int(options.tensors_are_uniform), basis, flags
+
+
+def assemble_gstc(mesh, options: GstcOptions, tensors, calderon=True, mass=True, out=None, scaled=True,
+                  params=None):
+    """eta0-scaled SIE-GSTC matrix (dim 4E). tensors: complex128 device tensor of
+    32 values per face (or 32 if options.tensors_are_uniform). scaled=False gives
+    the unscaled operator (assemble_Z_calderon_kernel + assemble_Z_mass_kernel,
+    sie_kernels_assembly.cuh:24-148, 282-452). params: kernel parameters to use
+    instead of those derived from options."""
+    dim = 4 * mesh.ne
+    p = calderon_params(options) if params is None else params
+    t, uni, basis, flags = _gstc_args(mesh, options, tensors, calderon, mass, scaled)
     Z = colmajor(dim) if out is None else out
     ld = _check_colmajor(Z, dim)
-    flags = (1 if calderon else 0) | (2 if mass else 0)
-    basis = {"face": 0, "spherical": 1}[options.basis]
-    _lib.check(_lib.lib.sie_assemble_gstc(mesh.handle, ctypes.byref(p), _dptr(t),
-                                          int(options.tensors_are_uniform), basis, flags,
+    _lib.check(_lib.lib.sie_assemble_gstc(mesh.handle, ctypes.byref(p), _dptr(t), uni, basis, flags,
                                           _dptr(Z), ld, _stream()))
     return Z
+
+
+def _vec(x, dim):
+    if x.shape != (dim,):
+        raise ValueError(f"expected a vector of length {dim}, got shape {tuple(x.shape)}")
+    return x.to(device="cuda", dtype=torch.complex128).contiguous()
+
+
+def gstc_matvec(mesh, options: GstcOptions, tensors, x, calderon=True, mass=True, scaled=True, out=None,
+                params=None):
+    """Matrix-free y = Z x with the SIE-GSTC operator of assemble_gstc (Z is not
+    stored). scaled=False applies the reference matvec's unscaled operator
+    (matvec_Z_calderon_kernel + matvec_Z_mass_kernel, sie_kernels_matvec.cuh:9-327).
+    params: kernel parameters to use instead of those derived from options."""
+    dim = 4 * mesh.ne
+    p = calderon_params(options) if params is None else params
+    t, uni, basis, flags = _gstc_args(mesh, options, tensors, calderon, mass, scaled)
+    xv = _vec(x, dim)
+    y = torch.empty(dim, dtype=torch.complex128, device="cuda") if out is None else out
+    _lib.check(_lib.lib.sie_gstc_matvec(mesh.handle, ctypes.byref(p), _dptr(t), uni, basis, flags,
+                                        _dptr(xv), _dptr(y), _stream()))
+    return y
+
+
+def pec_matvec(mesh, options: PecOptions, x, out=None):
+    """Matrix-free y = Z x with the T-EFIE operator of assemble_pec."""
+    dim = mesh.ne
+    p = calderon_params(options)
+    xv = _vec(x, dim)
+    y = torch.empty(dim, dtype=torch.complex128, device="cuda") if out is None else out
+    _lib.check(_lib.lib.sie_pec_matvec(mesh.handle, ctypes.byref(p), _dptr(xv), _dptr(y), _stream()))
+    return y
+
+
+def pmchwt_matvec(mesh, options: DielectricOptions, x, out=None):
+    """Matrix-free y = Z x with the T-PMCHWT operator of assemble_pmchwt."""
+    dim = 2 * mesh.ne
+    p = calderon_params(options)
+    xv = _vec(x, dim)
+    y = torch.empty(dim, dtype=torch.complex128, device="cuda") if out is None else out
+    _lib.check(_lib.lib.sie_pmchwt_matvec(mesh.handle, ctypes.byref(p), _dptr(xv), _dptr(y), _stream()))
+    return y
 
 
 def assemble_pec(mesh, options: PecOptions, out=None):

@@ -167,18 +167,19 @@ def _fl(p, names):
 _PNAMES = ("k1", "k2", "mu_r1", "mu_r2", "eps_r1", "eps_r2", "mu0", "eps0", "omega", "threshold")
 
 
-def oracle_gstc(mesh, p, tensors, uniform, basis, with_calderon=True, with_mass=True):
-    """Oracle GSTC matrix from a CalderonParams struct (float fields)."""
+def oracle_gstc(mesh, p, tensors, uniform, basis, with_calderon=True, with_mass=True, scaled=True):
+    """Oracle GSTC matrix from a CalderonParams struct (float fields);
+    scaled=False: the unscaled operator (eta0 = 1)."""
     lib = ctypes.CDLL(ORACLE_SO)
     ne, nf = mesh.ne, mesh.nf
     dim = 4 * ne
     Z = np.zeros((dim, dim), complex, order="F")
-    fn = lib.oracle_assemble_gstc
+    fn = lib.oracle_assemble_gstc_ex
     fn.argtypes = [_P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int] + [ctypes.c_float] * 10 + \
-        [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P]
+        [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P]
     t = np.ascontiguousarray(tensors, complex)
     fn(_ptr(mesh.vertices), _ptr(mesh.faces), _ptr(mesh.face_edges), nf, ne, p.rule, *_fl(p, _PNAMES),
-       _ptr(t), int(uniform), basis, int(with_calderon), int(with_mass), _ptr(Z))
+       _ptr(t), int(uniform), basis, int(with_calderon), int(with_mass), int(scaled), _ptr(Z))
     return Z
 
 

@@ -102,3 +102,39 @@ def test_tl_symmetry_and_dad_identity():
     assert np.linalg.norm(jm + mj) / np.linalg.norm(jm) < 1e-15
     ratio = (eta0 ** 2) * np.float64(np.float32(p.eps0)) / np.float64(np.float32(p.mu0))
     assert np.linalg.norm(mm - ratio * Jb) / np.linalg.norm(mm) < 1e-6
+
+
+def test_unscaled_operator_is_dad_of_scaled():
+    """The unscaled GSTC operator (eta0 = 1, the operator of the reference's
+    matrix-free matvec) and the eta0-scaled one satisfy Z_s = D Z_u D with
+    D = diag(1, eta0, 1, eta0), for the Calderon and the mass parts alike. The
+    mass part rounds eta0 in float, hence the 1e-6 bound."""
+    V, F, E, FE = ol.icosphere(1)
+
+    class Mm:
+        pass
+    mesh = Mm()
+    mesh.vertices, mesh.faces, mesh.edges, mesh.face_edges = V, F, E, FE
+    mesh.nf, mesh.ne = len(F), len(E)
+
+    class P:
+        pass
+    p = P()
+    eps0, mu0 = 8.8541878188e-12, 1.25663706127e-6
+    c0 = 1 / np.sqrt(mu0 * eps0)
+    f = 4.0 * c0 / (2 * np.pi)
+    om = 2 * np.pi * f
+    for k, v in dict(k1=om * np.sqrt(2.0 * mu0 * eps0), k2=om * np.sqrt(mu0 * eps0), mu_r1=1, mu_r2=1, eps_r1=2,
+                     eps_r2=1, mu0=mu0, eps0=eps0, omega=om, threshold=0.1 * c0 / f).items():
+        setattr(p, k, float(np.float32(v)))
+    p.rule = 5
+    t = ol.random_gstc_tensors(len(F), seed=11)
+    ne = len(E)
+    eta0 = np.sqrt(np.float64(np.float32(p.mu0) / np.float32(p.eps0)))
+    d = np.tile(np.repeat([1.0, eta0], ne), 2)
+    for cal, mass in ((True, False), (False, True), (True, True)):
+        Zs = ol.oracle_gstc(mesh, p, t, False, 1, cal, mass, scaled=True)
+        Zu = ol.oracle_gstc(mesh, p, t, False, 1, cal, mass, scaled=False)
+        assert np.linalg.norm(Zs - Zu) / np.linalg.norm(Zs) > 1e-2  # eta0 ~ 377 is really applied
+        dzd = d[:, None] * Zu * d[None, :]
+        assert np.linalg.norm(Zs - dzd) / np.linalg.norm(Zs) < 1e-6, (cal, mass)
