@@ -38,11 +38,11 @@ WORKLOADS = {
     "gstc-spindle-8k": ("spindle", 30, 7, 10),
 }
 
-PROF = {"gemm_f32": 0, "gemm_f64": 1, "far_pairs": 2, "panel": 3, "gemm_f16x3": 7}
+PROF = {"gemm_f32": 0, "gemm_f64": 1, "far_pairs": 2, "panel": 3, "gemm_f16x3": 7, "gemm_f16x3_wide": 8}
 # Published dense peaks. The split-FP16 GEMM issues three FP16 MFMA products per
 # algorithmic FP32 product, so its ceiling for algorithmic flops is 2.5 PF / 3.
 PEAK = {"gemm_f32": (157.3e12, "mfma"), "gemm_f64": (78.6e12, "mfma"), "far_pairs": (78.6e12, "valu-fp64"),
-        "panel": (8.0e12, "hbm"), "gemm_f16x3": (2.5e15 / 3, "mfma")}
+        "panel": (8.0e12, "hbm"), "gemm_f16x3": (2.5e15 / 3, "mfma"), "gemm_f16x3_wide": (2.5e15 / 3, "mfma")}
 
 
 DTYPE = {"f16x3": "c64 LU on split-fp16 MFMA (fp32-accurate) + c128 refinement",
@@ -123,7 +123,12 @@ def cpu_baseline(mesh, o, kind, seconds):
             "assembly_s": asm, "lu_s": lu_ext}
 
 
-TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "cgemm_f16x3_gstc-spindle-72k_traffic.txt")
+# counter summaries per profiled GEMM kernel: (kernel-name part, summary file)
+TRAFFIC = {
+    "gemm_f16x3_wide": ("cgemm_f16x3_r3_kernel",
+                        os.path.join(ROOT, "profiles", "cgemm_f16x3_r3_gstc-spindle-72k_traffic.txt")),
+    "gemm_f16x3": ("cgemm_f16x3_kernel", os.path.join(ROOT, "profiles", "cgemm_f16x3_gstc-spindle-72k_traffic.txt")),
+}
 
 
 def copy_calibration():
@@ -150,33 +155,40 @@ def _pmc_mean_kib(csv_path, counter, name_part):
 
 
 def summarize_pmc(fetch_csv, write_csv, cal_fetch_csv, cal_write_csv, command):
-    """Writes the traffic summary of the split-FP16 GEMM: HBM bytes per launch
-    from FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by what the
-    counter reads for a 256 MiB copy (elementwise copy kernel)."""
-    f, nf = _pmc_mean_kib(fetch_csv, "FETCH_SIZE", "cgemm_f16x3")
-    w, nw = _pmc_mean_kib(write_csv, "WRITE_SIZE", "cgemm_f16x3")
+    """Writes the traffic summaries of the split-FP16 GEMM kernels: HBM bytes per
+    launch from FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by what
+    the counter reads for a 256 MiB copy (elementwise copy kernel)."""
     # the runtime's blit kernel performs the 256 MiB device-to-device copy
     cf, _ = _pmc_mean_kib(cal_fetch_csv, "FETCH_SIZE", "copyBuffer")
     cw, _ = _pmc_mean_kib(cal_write_csv, "WRITE_SIZE", "copyBuffer")
     mib256 = 256 * 1024.0  # KiB
     ff, fw = mib256 / cf, mib256 / cw
-    traffic = (f * ff + w * fw) * 1024.0
-    with open(TRAFFIC_SUMMARY, "w") as out:
-        out.write("# HBM traffic of cgemm_f16x3_kernel (LU trailing update), gstc-spindle-72k, one iteration\n")
-        out.write(f"# command: {command}\n")
-        out.write(f"launches_fetch: {nf}\nlaunches_write: {nw}\n")
-        out.write(f"fetch_kib_per_launch_raw: {f:.1f}\nwrite_kib_per_launch_raw: {w:.1f}\n")
-        out.write(f"copy_256MiB_fetch_kib: {cf:.1f}\ncopy_256MiB_write_kib: {cw:.1f}\n")
-        out.write(f"fetch_scale: {ff:.4f}\nwrite_scale: {fw:.4f}\n")
-        out.write(f"traffic_bytes_per_launch: {traffic:.6e}\n")
-    print(open(TRAFFIC_SUMMARY).read())
+    for name_part, path in TRAFFIC.values():
+        try:
+            f, nf = _pmc_mean_kib(fetch_csv, "FETCH_SIZE", name_part)
+            w, nw = _pmc_mean_kib(write_csv, "WRITE_SIZE", name_part)
+        except SystemExit as e:
+            print(e)
+            continue
+        traffic = (f * ff + w * fw) * 1024.0
+        with open(path, "w") as out:
+            out.write(f"# HBM traffic of {name_part} (LU trailing update), gstc-spindle-72k, one iteration\n")
+            out.write(f"# command: {command}\n")
+            out.write(f"launches_fetch: {nf}\nlaunches_write: {nw}\n")
+            out.write(f"fetch_kib_per_launch_raw: {f:.1f}\nwrite_kib_per_launch_raw: {w:.1f}\n")
+            out.write(f"copy_256MiB_fetch_kib: {cf:.1f}\ncopy_256MiB_write_kib: {cw:.1f}\n")
+            out.write(f"fetch_scale: {ff:.4f}\nwrite_scale: {fw:.4f}\n")
+            out.write(f"traffic_bytes_per_launch: {traffic:.6e}\n")
+        print(open(path).read())
 
 
-def traffic_per_launch():
-    """HBM bytes per launch of the dominant kernel from the committed counter
+def traffic_per_launch(kernel):
+    """HBM bytes per launch of a profiled GEMM kernel from its committed counter
     summary, or None."""
+    if kernel not in TRAFFIC:
+        return None
     try:
-        for line in open(TRAFFIC_SUMMARY):
+        for line in open(TRAFFIC[kernel][1]):
             if line.startswith("traffic_bytes_per_launch:"):
                 return float(line.split(":")[1])
     except OSError:
@@ -195,6 +207,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--workload", default="gstc-spindle-72k", choices=sorted(WORKLOADS))
     ap.add_argument("--variant", default="f16x3", choices=["4m", "3m", "f16x3"])
+    ap.add_argument("--ir-method", default="fgmres", choices=["ir", "fgmres"],
+                    help="FP64 refinement around the FP32 LU: classic sweeps or FGMRES")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison path")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -224,7 +238,8 @@ def main():
     log(f"workload {args.workload}: F={mesh.nf} E={mesh.ne} N={N} rule={rule} params/face={kind}")
     rng = np.random.default_rng(20260101)
     params = torch.from_numpy(rng.uniform(-2, 2, size=(mesh.nf, kind)) / k0).cuda()
-    problem = ad.GstcProblem(mesh, o, waves, kind, mode="mixed", ir_tol=1e-12, ir_maxit=40, variant=args.variant)
+    problem = ad.GstcProblem(mesh, o, waves, kind, mode="mixed", ir_tol=1e-12, ir_maxit=40, variant=args.variant,
+                             ir_method=args.ir_method)
     lib = _lib.lib
     # load every kernel once on a tiny problem (code objects, workspaces)
     tiny = sm.Mesh.icosphere(1.0, 1)
@@ -265,7 +280,7 @@ def main():
         lib.sie_profile_enable(k, 0)
     phases = {k: v / args.steps for k, v in problem.timings.items()}
     log(f"mixed: {s_per_iter:.3f} s/iter  phases {json.dumps({k: round(v, 3) for k, v in phases.items()})}")
-    log(f"IR: rel residual {ir_res:.2e} in {ir_its} sweeps; J={float(J):.6e}")
+    log(f"{args.ir_method}: rel residual {ir_res:.2e} in {ir_its} steps; J={float(J):.6e}")
 
     # re-run the last iterate's solve in FP64 for the error and the speed ratio
     out_extra = {}
@@ -312,11 +327,11 @@ def main():
             rooflines[name] = {"bound": bound, "achieved": ach / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
                                "frac": ach / peak, "launches": n, "avg_ms": ms / n, "total_ms": ms,
                                "alg_bytes_per_launch": by / n}
-    dom = max(("gemm_f32", "gemm_f16x3", "far_pairs"), key=lambda k: prof[k][1])
+    dom = max(("gemm_f32", "gemm_f16x3", "gemm_f16x3_wide", "far_pairs"), key=lambda k: prof[k][1])
     r = dict(rooflines.get(dom, {}))
     roof = {"kernel": dom, "bound": "valu-fp64" if dom == "far_pairs" else "mfma",
             "achieved": r.get("achieved"), "peak": r.get("peak"), "unit": "TFLOP/s", "frac": r.get("frac"),
-            "traffic": traffic_per_launch() if dom == "gemm_f16x3" else None, "avg_ms": r.get("avg_ms"),
+            "traffic": traffic_per_launch(dom), "avg_ms": r.get("avg_ms"),
             "launches": r.get("launches"), "alg_bytes_per_launch": r.get("alg_bytes_per_launch")}
 
     cpu = None
@@ -331,7 +346,8 @@ def main():
         "vs_baseline": None, "dtype": DTYPE[args.variant], "data": "synthetic (seeded parameters, generated mesh)",
         "config": {"workload": args.workload, "faces": mesh.nf, "edges": mesh.ne, "unknowns": N,
                    "quadrature_rule": rule, "params_per_face": kind, "rhs": 1, "lu_variant": args.variant,
-                   "objective": "far-field power, 10 deg cone about +z", "ir_tol": 1e-12},
+                   "objective": "far-field power, 10 deg cone about +z", "ir_tol": 1e-12,
+                   "ir_method": args.ir_method},
         "phases_s": {k: round(v, 4) for k, v in phases.items()},
         "ir_rel_residual": ir_res, "ir_sweeps": ir_its,
         "roofline": roof, "rooflines": rooflines, "cpu_baseline": cpu,

@@ -196,6 +196,19 @@ int sie_ir_solve(int op, int n, int nrhs, const double* Z_dev, long long ldz, co
                  long long ldx, double tol, int maxit, double* rel_residual, int* iterations,
                  void* stream);
 
+/* The same solve by GMRES(restart) right-preconditioned with the FP32 LU
+ * (GMRES-IR): z_k = LU32^{-1} v_k with FP64 arithmetic on the FP32 factors,
+ * w = op(Z) z_k and its CGS2 orthogonalisation in FP64, x += [z_0..z_{k-1}] y;
+ * each cycle ends with the true FP64 residual. Same stopping rule as
+ * sie_ir_solve; iterations[c] counts inner steps (one preconditioner solve
+ * and one FP64 GEMV each). No counterpart in the reference. Synchronises the
+ * stream every step. */
+int sie_fgmres_ir_solve(int op, int n, int nrhs, const double* Z_dev, long long ldz,
+                        const float* LU_dev, long long ldl, const int* ipiv_dev,
+                        const double* B_dev, long long ldb, double* X_dev, long long ldx,
+                        double tol, int maxit, int restart, double* rel_residual, int* iterations,
+                        void* stream);
+
 /* Euclidean norm of n complex doubles (deterministic two-stage reduction,
  * synchronises the stream; returns -1 on error). */
 double sie_norm2(long long n, const double* x_dev, void* stream);
@@ -271,7 +284,8 @@ int sie_gstc_param_grad(int nf, int kind, const double* records_dev, const doubl
 /* Per-kernel timing with HIP events on the launch stream (no counterpart in
  * the reference, which times phases with std::chrono). kernel: 0 FP32 GEMM,
  * 1 FP64 GEMM, 2 far-pair assembly, 3 LU panel, 4 near-pair assembly,
- * 5 per-edge gather, 6 GEMV, 7 split-FP16 GEMM. enable(k, 1) clears and
+ * 5 per-edge gather, 6 GEMV, 7 split-FP16 GEMM (128 x 128 tiles), 8 split-FP16
+ * GEMM (256 x 128 tiles, K >= 512 and N > 1024). enable(k, 1) clears and
  * starts recording; read() synchronises on the recorded events and returns
  * the launch count, summed duration and summed algorithmic flops / bytes. */
 int sie_profile_enable(int kernel, int on);

@@ -182,6 +182,101 @@ int gmres_run(int n, int m, int maxnit, double tol, sie_matvec_fn matvec, void* 
     return info;
 }
 
+// GMRES(m) refinement of op(Z) x = b (Z complex double) right-preconditioned
+// by an FP32 LU of Z (GMRES-IR, Carson & Higham, SIAM J. Sci. Comput. 40
+// (2018) A817-A847): z_k = LU32^{-1} v_k is applied with FP64 arithmetic to the
+// FP32 factors, so the preconditioner is one fixed linear operator and the
+// Krylov space removes the few slow error components of an FP32
+// factorisation in a handful of steps instead of one refinement sweep each.
+// w = op(Z) z_k is orthogonalised in FP64 (CGS2) and the z_k are kept
+// (flexible form), so x += [z_0 .. z_{k-1}] y satisfies the FP64 Arnoldi
+// relation. Every cycle ends with the true residual b - op(Z) x. maxit bounds
+// the total number of inner steps (one preconditioner solve and one FP64 GEMV each).
+int fgmres_ir_run(int op, int n, const double* Z, long long ldz, const float* LU, long long ldl, const int* ipiv,
+                  const double* b, double* x, double tol, int m, int maxit, double* out_rel, hipStream_t s) {
+    require(op == 0 || op == 2, "op must be 0 (N) or 2 (C)");
+    require(n > 0 && m > 0 && maxit > 0, "invalid FGMRES-IR dimensions");
+    const double bn = sie_norm2(n, b, s);
+    if (bn < 0) throw Error(SIE_ERR_DEVICE, sie_last_error());
+    if (!(bn > 0) || !std::isfinite(bn)) throw Error(SIE_ERR_NONFINITE, "right-hand side norm must be finite and nonzero");
+    const size_t n2 = (size_t)2 * n;
+    DevBuf<double> V(n2 * (m + 1)), Zb(n2 * m), r(n2), zt(n2), h((size_t)2 * (m + 1)), h2((size_t)2 * (m + 1));
+    const int ldH = m + 1;
+    std::vector<zc> H((size_t)ldH * m), sn(m), g(ldH), y(m), hv(m + 1), hv2(m + 1);
+    std::vector<double> cs(m);
+    const unsigned gb = blocks_for(n);
+    SIE_HIP(hipMemsetAsync(x, 0, sizeof(double) * n2, s));
+    SIE_HIP(hipMemcpyAsync(r.p, b, sizeof(double) * n2, hipMemcpyDeviceToDevice, s));
+    double beta = bn;
+    int total = 0;
+    while (beta > tol * bn && total < maxit) {
+        gm_copy_scaled_kernel<<<gb, 256, 0, s>>>(n, r.p, beta, V.p);
+        SIE_HIP(hipGetLastError());
+        std::fill(g.begin(), g.end(), zc(0));
+        g[0] = beta;
+        int k = 0;
+        for (; k < m && total < maxit; ++k) {
+            ++total;
+            double* vk = V.p + n2 * k;
+            double* zk = Zb.p + n2 * k;
+            double* w = V.p + n2 * (k + 1);
+            // z_k = LU32^{-1} v_k with FP64 arithmetic, through a fixed buffer
+            // so that the captured sweep sequence is reused every step
+            SIE_HIP(hipMemcpyAsync(zt.p, vk, sizeof(double) * n2, hipMemcpyDeviceToDevice, s));
+            lu_solve_f32_f64(op, n, LU, ldl, ipiv, zt.p, s);
+            SIE_HIP(hipMemcpyAsync(zk, zt.p, sizeof(double) * n2, hipMemcpyDeviceToDevice, s));
+            check(sie_zgemv(op, n, n, 1.0, Z, ldz, zk, 0.0, w, s));
+            check(sie_zgemv(2, n, k + 1, 1.0, V.p, n, w, 0.0, h.p, s));
+            check(sie_zgemv(0, n, k + 1, -1.0, V.p, n, h.p, 1.0, w, s));
+            check(sie_zgemv(2, n, k + 1, 1.0, V.p, n, w, 0.0, h2.p, s));
+            check(sie_zgemv(0, n, k + 1, -1.0, V.p, n, h2.p, 1.0, w, s));
+            SIE_HIP(hipMemcpyAsync(hv.data(), h.p, sizeof(zc) * (k + 1), hipMemcpyDeviceToHost, s));
+            SIE_HIP(hipMemcpyAsync(hv2.data(), h2.p, sizeof(zc) * (k + 1), hipMemcpyDeviceToHost, s));
+            const double hn = sie_norm2(n, w, s);  // synchronises the stream
+            if (!std::isfinite(hn) || hn < 0) throw Error(SIE_ERR_NONFINITE, "FGMRES-IR Arnoldi vector is not finite");
+            zc* col = &H[(size_t)k * ldH];
+            for (int j = 0; j <= k; ++j) col[j] = hv[j] + hv2[j];
+            col[k + 1] = hn;
+            if (hn > 0) {
+                gm_scale_kernel<<<gb, 256, 0, s>>>(n, 1.0 / hn, w);
+                SIE_HIP(hipGetLastError());
+            }
+            for (int i = 0; i < k; ++i) {
+                const zc a = col[i], c1 = col[i + 1];
+                col[i] = cs[i] * a + sn[i] * c1;
+                col[i + 1] = -std::conj(sn[i]) * a + cs[i] * c1;
+            }
+            givens(col[k], col[k + 1], cs[k], sn[k]);
+            col[k] = cs[k] * col[k] + sn[k] * col[k + 1];
+            col[k + 1] = 0.0;
+            const zc gk = g[k];
+            g[k] = cs[k] * gk;
+            g[k + 1] = -std::conj(sn[k]) * gk;
+            if (std::abs(g[k + 1]) <= tol * bn || hn == 0) {
+                ++k;
+                break;
+            }
+        }
+        // x += Zk y with H(0:k, 0:k) y = g(0:k)
+        for (int i = k - 1; i >= 0; --i) {
+            zc acc = g[i];
+            for (int j = i + 1; j < k; ++j) acc -= H[(size_t)j * ldH + i] * y[j];
+            y[i] = acc / H[(size_t)i * ldH + i];
+        }
+        if (k > 0) {
+            SIE_HIP(hipMemcpyAsync(h.p, y.data(), sizeof(zc) * k, hipMemcpyHostToDevice, s));
+            check(sie_zgemv(0, n, k, 1.0, Zb.p, n, h.p, 1.0, x, s));
+        }
+        // true residual r = b - op(Z) x
+        SIE_HIP(hipMemcpyAsync(r.p, b, sizeof(double) * n2, hipMemcpyDeviceToDevice, s));
+        check(sie_zgemv(op, n, n, -1.0, Z, ldz, x, 1.0, r.p, s));
+        beta = sie_norm2(n, r.p, s);
+        if (!std::isfinite(beta) || beta < 0) throw Error(SIE_ERR_NONFINITE, "FGMRES-IR diverged (non-finite residual)");
+    }
+    if (out_rel) *out_rel = beta / bn;
+    return total;
+}
+
 namespace {
 struct DenseOp {
     const double* Z;
@@ -207,6 +302,31 @@ int sie_gmres(int n, int m, int maxnit, double tol, sie_matvec_fn matvec, void* 
         const int r = gmres_run(n, m, maxnit, tol, matvec, user, b_dev, x_dev, out_res, out_iters, history,
                                 history_cap, history_len, (hipStream_t)stream);
         if (info) *info = r;
+        set_last_error("");
+        return SIE_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return SIE_ERR_DEVICE;
+    }
+}
+
+int sie_fgmres_ir_solve(int op, int n, int nrhs, const double* Z, long long ldz, const float* LU, long long ldl,
+                        const int* ipiv, const double* B, long long ldb, double* X, long long ldx, double tol,
+                        int maxit, int restart, double* rel_residual, int* iterations, void* stream) {
+    try {
+        require(n > 0 && nrhs > 0 && ldz >= n && ldl >= n && ldb >= n && ldx >= n, "invalid FGMRES-IR dimensions");
+        require(Z && LU && ipiv && B && X, "null FGMRES-IR argument");
+        require(tol > 0 && std::isfinite(tol), "FGMRES-IR tolerance must be positive");
+        for (int c = 0; c < nrhs; ++c) {
+            double rel = 0;
+            const int it = fgmres_ir_run(op, n, Z, ldz, LU, ldl, ipiv, B + 2 * (long long)c * ldb,
+                                         X + 2 * (long long)c * ldx, tol, restart, maxit, &rel, (hipStream_t)stream);
+            if (rel_residual) rel_residual[c] = rel;
+            if (iterations) iterations[c] = it;
+        }
         set_last_error("");
         return SIE_OK;
     } catch (const Error& e) {

@@ -329,6 +329,46 @@ __global__ void split_f16_kernel(int R, int K, const c32* __restrict__ X, long l
     }
 }
 
+// The same split into the tile-blocked layout of cgemm_f16x3_r3_kernel:
+// P[blk][kb][plane][part][rr][8] with rows r = blk RB + rr padded with zeros to
+// whole blocks, so that one tile's K slice (all four planes) is one contiguous
+// RB x 128-byte piece and its whole K panel one contiguous range. One thread
+// per (row, 8-wide k group) writes a 16-byte vector to each plane.
+__global__ void split_f16_blk_kernel(int R, int K, int RB, const c32* __restrict__ X, long long ld, int trans,
+                                     const unsigned int* __restrict__ mx, _Float16* __restrict__ P) {
+    const int KB = (K + 15) / 16, nblk = (R + RB - 1) / RB;
+    const long long total = (long long)nblk * RB * KB * 2;
+    const float sc = pow2_scale(*mx);
+    const float hs = trans ? kLoShift : 1.f;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        // rows fastest: consecutive lanes read consecutive rows of a column of X
+        const int rr = (int)(t % RB);
+        long long rest = t / RB;
+        const int part = (int)(rest & 1);
+        rest >>= 1;
+        const int kb = (int)(rest % KB);
+        const int blk = (int)(rest / KB);
+        const int r = blk * RB + rr;
+        half8 h[4];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int k = kb * 16 + part * 8 + e;
+            c32 v = {0.f, 0.f};
+            if (r < R && k < K) v = trans ? X[(long long)r * ld + k] : X[(long long)k * ld + r];
+            const float xr = v.re * sc, xi = v.im * sc;
+            const _Float16 rh = (_Float16)xr, ih = (_Float16)xi;
+            h[0][e] = (_Float16)((float)rh * hs);
+            h[1][e] = (_Float16)((xr - (float)rh) * kLoShift);
+            h[2][e] = (_Float16)((float)ih * hs);
+            h[3][e] = (_Float16)((xi - (float)ih) * kLoShift);
+        }
+        half8* out = (half8*)P + (((long long)blk * KB + kb) * 8) * RB + (long long)part * RB + rr;
+#pragma unroll
+        for (int pl = 0; pl < 4; ++pl) out[(long long)pl * 2 * RB] = h[pl];
+    }
+}
+
 // Tile of a linear block id. Work-groups are dispatched round-robin over the
 // 8 XCDs, each with its own L2: give every XCD a contiguous range of tiles and
 // walk that range in groups of GM tile rows, column by column, so that the
@@ -396,6 +436,41 @@ __device__ __forceinline__ void f16x3_acc_store(const floatx16 (&accR)[TI][2], c
                 const long long j = jb + tj * 32 + 8 * (v >> 2) + 4 * part + (v & 3);
                 if (i < M && j < N) C[j * ldc + i] = c32{f * accR[ti][tj][v], f * accI[ti][tj][v]};
             }
+}
+
+// One K slice of 16: the 4M complex product of the wave tile, 3 MFMAs per
+// real product (hi*hi, hi*lo, lo*hi) with the small terms first.
+template <int TI>
+__device__ __forceinline__ void f16x3_mfma(const half8 (&a)[TI][4], const half8 (&b)[2][4], floatx16 (&accR)[TI][2],
+                                           floatx16 (&accI)[TI][2]) {
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj) {
+        // B hi planes carry 2^11: b?2 = h 2^11, b?l = l 2^11, b? = h
+        const half8 br2 = b[tj][0], brl = b[tj][1], bi2 = b[tj][2], bil = b[tj][3];
+        const half8 br = unshift8(br2), bi = unshift8(bi2);
+#pragma unroll
+        for (int ti = 0; ti < TI; ++ti) {
+            const half8 arh = a[ti][0], arl = a[ti][1], aih = a[ti][2], ail = a[ti][3];
+            const half8 naih = neg8(aih), nail = neg8(ail);
+            floatx16 cr = accR[ti][tj], ci = accI[ti][tj];
+            // re: ar br - ai bi, small terms first
+            cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(brl, arh, cr, 0, 0, 0);
+            cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(br, arl, cr, 0, 0, 0);
+            cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(bil, naih, cr, 0, 0, 0);
+            cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(bi, nail, cr, 0, 0, 0);
+            cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(br2, arh, cr, 0, 0, 0);
+            cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(bi2, naih, cr, 0, 0, 0);
+            // im: ar bi + ai br
+            ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(bil, arh, ci, 0, 0, 0);
+            ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(bi, arl, ci, 0, 0, 0);
+            ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(brl, aih, ci, 0, 0, 0);
+            ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(br, ail, ci, 0, 0, 0);
+            ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(bi2, arh, ci, 0, 0, 0);
+            ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(br2, aih, ci, 0, 0, 0);
+            accR[ti][tj] = cr;
+            accI[ti][tj] = ci;
+        }
+    }
 }
 
 #define SIE_GPTR(p) ((const void __attribute__((address_space(1)))*)(p))
@@ -472,36 +547,90 @@ __global__ __launch_bounds__(256, MINB) void cgemm_f16x3_kernel(int M, int N, in
 #pragma unroll
             for (int pl = 0; pl < 4; ++pl) b[t][pl] = sB[buf][pl][part][wn * 64 + t * 32 + r32];
         if (kb + 1 < KB) dma(kb + 1, buf ^ 1);
-#pragma unroll
-        for (int tj = 0; tj < 2; ++tj) {
-            // B hi planes carry 2^11: b?2 = h 2^11, b?l = l 2^11, b? = h
-            const half8 br2 = b[tj][0], brl = b[tj][1], bi2 = b[tj][2], bil = b[tj][3];
-            const half8 br = unshift8(br2), bi = unshift8(bi2);
-#pragma unroll
-            for (int ti = 0; ti < TI; ++ti) {
-                const half8 arh = a[ti][0], arl = a[ti][1], aih = a[ti][2], ail = a[ti][3];
-                const half8 naih = neg8(aih), nail = neg8(ail);
-                floatx16 cr = accR[ti][tj], ci = accI[ti][tj];
-                // re: ar br - ai bi, small terms first
-                cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(brl, arh, cr, 0, 0, 0);
-                cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(br, arl, cr, 0, 0, 0);
-                cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(bil, naih, cr, 0, 0, 0);
-                cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(bi, nail, cr, 0, 0, 0);
-                cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(br2, arh, cr, 0, 0, 0);
-                cr = __builtin_amdgcn_mfma_f32_32x32x16_f16(bi2, naih, cr, 0, 0, 0);
-                // im: ar bi + ai br
-                ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(bil, arh, ci, 0, 0, 0);
-                ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(bi, arl, ci, 0, 0, 0);
-                ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(brl, aih, ci, 0, 0, 0);
-                ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(br, ail, ci, 0, 0, 0);
-                ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(bi2, arh, ci, 0, 0, 0);
-                ci = __builtin_amdgcn_mfma_f32_32x32x16_f16(br2, aih, ci, 0, 0, 0);
-                accR[ti][tj] = cr;
-                accI[ti][tj] = ci;
-            }
-        }
+        f16x3_mfma<TI>(a, b, accR, accI);
     }
     f16x3_acc_store<TI>(accR, accI, C, ldc, M, N, i0 + wm * 32 * TI, j0 + wn * 64, lane & 31, lane >> 5, alpha / unit);
+}
+
+// Block tile 256 x 128, eight waves in a 4 x 2 grid, wave tile 64 x 64, one
+// workgroup per CU. Three LDS stages (48 KB each, 144 KB) let the loads of
+// slice k + 2 start while slice k is multiplied, two slices of MFMAs ahead of
+// their use; each stage is its own LDS object so that the compiler's wait
+// for outstanding direct-to-LDS loads before an LDS read can be limited to
+// the loads into that stage. The wave tile and the per-slice product are
+// those of cgemm_f16x3_kernel; a 256-row tile halves the B-panel traffic per
+// MFMA of the 128-row tile.
+struct F16Stage {
+    half8 a[4][2][256];
+    half8 b[4][2][128];
+};
+
+template <bool SWZ>
+__global__ __launch_bounds__(512, 1) void cgemm_f16x3_r3_kernel(int M, int N, int K, float alpha, float beta,
+                                                               const _Float16* __restrict__ PA,
+                                                               const _Float16* __restrict__ PB,
+                                                               const unsigned int* __restrict__ mx,
+                                                               c32* __restrict__ C, long long ldc) {
+    constexpr int BM = 256, BN = 128, TI = 2;
+    constexpr int LA = BM * 8 / 512, LB = BN * 8 / 512;  // 16-byte pieces per thread and slice
+    __shared__ F16Stage st0, st1, st2;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 3, wn = wave >> 2;
+    int tm, tn;
+    f16x3_tile<SWZ>(blockIdx.x, (M + BM - 1) / BM, (N + BN - 1) / BN, tm, tn);
+    const int i0 = tm * BM, j0 = tn * BN;
+    const int KB = (K + 15) / 16;
+    // Operands in the tile-blocked layout of split_f16_blk_kernel: the K slice
+    // of this tile is 32 KB (A) and 16 KB (B) of contiguous bytes whose order
+    // is the LDS order, so thread t copies bytes 16 (q 512 + t). 32-bit byte
+    // offsets: the caller keeps each split operand below 4 GiB.
+    const unsigned sliceA = BM * 128u, sliceB = BN * 128u;
+    const unsigned baseA = (unsigned)tm * KB * sliceA + tid * 16u, baseB = (unsigned)tn * KB * sliceB + tid * 16u;
+    const char* pa8 = (const char*)PA;
+    const char* pb8 = (const char*)PB;
+    auto dma = [&](F16Stage& d, int kb) {
+        const unsigned oa = baseA + (unsigned)kb * sliceA, ob = baseB + (unsigned)kb * sliceB;
+        half8* la = &d.a[0][0][0] + wave * 64;
+        half8* lb = &d.b[0][0][0] + wave * 64;  // lane q lands at base + 16 q bytes
+#pragma unroll
+        for (int q = 0; q < LA; ++q)
+            __builtin_amdgcn_global_load_lds(SIE_GPTR(pa8 + (oa + q * 8192u)), SIE_LPTR(la + q * 512), 16, 0, 0);
+#pragma unroll
+        for (int q = 0; q < LB; ++q)
+            __builtin_amdgcn_global_load_lds(SIE_GPTR(pb8 + (ob + q * 8192u)), SIE_LPTR(lb + q * 512), 16, 0, 0);
+    };
+    floatx16 accR[TI][2], accI[TI][2];
+    const float unit = pow2_scale(mx[0]) * pow2_scale(mx[1]) * kLoShift;
+    f16x3_acc_init<TI>(accR, accI, C, ldc, M, N, i0 + wm * 64, j0 + wn * 64, lane & 31, lane >> 5,
+                       beta * unit / alpha, beta != 0.f);
+    const int part = lane >> 5, r32 = lane & 31;
+    // slice kb from stage `cur`; the loads of slice kb + 2 go to `nxt2` (the
+    // stage read by slice kb - 1, which every wave finished before this barrier)
+    auto step = [&](F16Stage& cur, F16Stage& nxt2, int kb) {
+        if (kb + 1 < KB)  // slice kb + 1's loads may stay in flight
+            asm volatile("s_waitcnt vmcnt(%0)\n\ts_barrier" ::"n"(LA + LB) : "memory");
+        else
+            asm volatile("s_waitcnt vmcnt(0)\n\ts_barrier" ::: "memory");
+        half8 a[TI][4], b[2][4];
+#pragma unroll
+        for (int t = 0; t < TI; ++t)
+#pragma unroll
+            for (int pl = 0; pl < 4; ++pl) a[t][pl] = cur.a[pl][part][wm * 64 + t * 32 + r32];
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int pl = 0; pl < 4; ++pl) b[t][pl] = cur.b[pl][part][wn * 64 + t * 32 + r32];
+        if (kb + 2 < KB) dma(nxt2, kb + 2);
+        f16x3_mfma<TI>(a, b, accR, accI);
+    };
+    dma(st0, 0);
+    if (KB > 1) dma(st1, 1);
+    for (int kb = 0; kb < KB; kb += 3) {
+        step(st0, st2, kb);
+        if (kb + 1 < KB) step(st1, st0, kb + 1);
+        if (kb + 2 < KB) step(st2, st1, kb + 2);
+    }
+    f16x3_acc_store<TI>(accR, accI, C, ldc, M, N, i0 + wm * 64, j0 + wn * 64, lane & 31, lane >> 5, alpha / unit);
 }
 
 // =================================================================== panel
@@ -878,15 +1007,20 @@ constexpr int TB = 64;
 // Inverse of every B x B diagonal block of the unit lower (which = 0) or upper
 // (which = 1) factor, one thread per column; inv[blk][c B + i] = X(i, c).
 // Partial last blocks are padded with the identity.
-template <typename T, int B>
-__global__ void diag_inv_kernel(int n, const cx<T>* __restrict__ LU, long long lda, int which,
+// Factors in TF, inverse computed and stored in T (T = double over FP32
+// factors gives the exact inverse of the FP32 blocks to FP64 rounding).
+template <typename TF, typename T, int B>
+__global__ void diag_inv_kernel(int n, const cx<TF>* __restrict__ LU, long long lda, int which,
                                 cx<T>* __restrict__ inv) {
     const int blk = blockIdx.x, c = threadIdx.x;
     const int k0 = blk * B;
     const int w = min(B, n - k0);
     cx<T>* X = inv + (size_t)blk * B * B;
     if (c >= B) return;
-    auto A = [&](int i, int j) { return LU[(long long)(k0 + j) * lda + k0 + i]; };
+    auto A = [&](int i, int j) {
+        const cx<TF> a = LU[(long long)(k0 + j) * lda + k0 + i];
+        return cx<T>{(T)a.re, (T)a.im};
+    };
     for (int i = 0; i < B; ++i) X[c * B + i] = {0, 0};
     if (c >= w) {
         X[c * B + c] = {1, 0};
@@ -970,8 +1104,8 @@ __global__ void conj_transpose_blocks_kernel(long long total, const cx<T>* __res
     }
 }
 
-template <typename T, int B>
-__global__ __launch_bounds__(256) void trsv_update_kernel(int w, int k0, const cx<T>* __restrict__ LU, long long lda,
+template <typename TF, typename T, int B>
+__global__ __launch_bounds__(256) void trsv_update_kernel(int w, int k0, const cx<TF>* __restrict__ LU, long long lda,
                                                           const cx<T>* __restrict__ Y, int op, cx<T>* __restrict__ vec,
                                                           int r0, int r1, int nxt0, int nw,
                                                           const cx<T>* __restrict__ invn, cx<T>* __restrict__ out) {
@@ -990,12 +1124,13 @@ __global__ __launch_bounds__(256) void trsv_update_kernel(int w, int k0, const c
         const int row = t % B, grp = t / B, r = rb + row;
         T ar = 0, ai = 0;
         if (r < r1) {
-            const cx<T>* a = LU + (long long)(k0 + grp * CG) * lda + r;
+            const cx<TF>* a = LU + (long long)(k0 + grp * CG) * lda + r;
 #pragma unroll 8
             for (int cc = 0; cc < CG; ++cc) {
                 const int c = grp * CG + cc;
                 if (c < w) {
-                    const cx<T> v = a[(long long)cc * lda];
+                    const cx<TF> f = a[(long long)cc * lda];
+                    const cx<T> v = {(T)f.re, (T)f.im};
                     ar += v.re * sy[c].re - v.im * sy[c].im;
                     ai += v.re * sy[c].im + v.im * sy[c].re;
                 }
@@ -1024,12 +1159,13 @@ __global__ __launch_bounds__(256) void trsv_update_kernel(int w, int k0, const c
             const int row = pass * RP + rr, r = rb + row;
             T ar = 0, ai = 0;
             if (r < r1) {
-                const cx<T>* a = LU + (long long)r * lda + k0 + sub * 4;
+                const cx<TF>* a = LU + (long long)r * lda + k0 + sub * 4;
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     const int c = sub * 4 + e;
                     if (c < w) {
-                        const cx<T> v = a[e];
+                        const cx<TF> f = a[e];
+                        const cx<T> v = {(T)f.re, (T)f.im};
                         ar += v.re * sy[c].re + v.im * sy[c].im;
                         ai += v.re * sy[c].im - v.im * sy[c].re;
                     }
@@ -1124,7 +1260,29 @@ static void gemm_f16x3(int M, int N, int K, double alpha, const c32* A, long lon
         return;
     }
     const int KB = (K + 15) / 16;
-    const size_t pa = (size_t)KB * M * 16 * 4, pb = (size_t)KB * N * 16 * 4;
+    // wide products (the LU's super-panel updates) use the 256 x 128 three-stage
+    // kernel; narrow ones (in-panel updates, N <= 1024) keep more tiles with 128 x 128
+    const bool r3 = K >= 512 && N > 1024;
+    if (r3) {
+        // the kernel addresses each split operand with 32-bit byte offsets:
+        // split larger products into row / column chunks below 4 GiB each
+        const long long per_row = (long long)KB * 128;  // bytes per padded row of a split operand
+        const long long cap = ((1LL << 32) - 1) / per_row / 256 * 256;
+        require(cap >= 256, "split-FP16 GEMM: K too large");
+        if (M > cap) {
+            for (long long r0 = 0; r0 < M; r0 += cap)
+                gemm_f16x3((int)std::min<long long>(cap, M - r0), N, K, alpha, A + r0, lda, B, ldb, beta, C + r0, ldc, s);
+            return;
+        }
+        if (N > cap) {
+            for (long long c0 = 0; c0 < N; c0 += cap)
+                gemm_f16x3(M, (int)std::min<long long>(cap, N - c0), K, alpha, A, lda, B + c0 * ldb, ldb, beta,
+                           C + c0 * ldc, ldc, s);
+            return;
+        }
+    }
+    // planes padded to whole 256-row (A) and 128-column (B) blocks
+    const size_t pa = (size_t)KB * ((M + 255) / 256 * 256) * 16 * 4, pb = (size_t)KB * ((N + 127) / 128 * 128) * 16 * 4;
     unsigned char* w = (unsigned char*)ws(6).get((pa + pb) * sizeof(_Float16) + 256);
     _Float16* PA = (_Float16*)w;
     _Float16* PB = PA + pa;
@@ -1133,19 +1291,35 @@ static void gemm_f16x3(int M, int N, int K, double alpha, const c32* A, long lon
     const long long ta = (long long)M * K, tb = (long long)K * N;
     maxabs_kernel<<<(unsigned)std::min<long long>((ta + 255) / 256, 1024), 256, 0, s>>>(M, K, A, lda, mx);
     maxabs_kernel<<<(unsigned)std::min<long long>((tb + 255) / 256, 1024), 256, 0, s>>>(K, N, B, ldb, mx + 1);
-    const long long sa = (long long)KB * M * 16, sb = (long long)KB * N * 16;
-    split_f16_kernel<<<(unsigned)std::min<long long>((sa + 255) / 256, 1 << 16), 256, 0, s>>>(M, K, A, lda, 0, mx,
-                                                                                             PA);
-    split_f16_kernel<<<(unsigned)std::min<long long>((sb + 255) / 256, 1 << 16), 256, 0, s>>>(N, K, B, ldb, 1,
-                                                                                             mx + 1, PB);
-    const long long tiles = (long long)((M + 127) / 128) * ((N + 127) / 128);
-    require(tiles * 256 < (1LL << 32), "split-FP16 GEMM: too many tiles for one dispatch");
-    prof().begin(PROF_GEMM_F16X3, s);
-    cgemm_f16x3_kernel<2, 2, true><<<(unsigned)tiles, 256, 0, s>>>(M, N, K, (float)alpha, (float)beta, PA, PB, mx, C,
-                                                                   ldc);
+    if (r3) {
+        const long long ra = (long long)((M + 255) / 256) * 256 * KB * 2, rb = (long long)((N + 127) / 128) * 128 * KB * 2;
+        split_f16_blk_kernel<<<(unsigned)std::min<long long>((ra + 255) / 256, 1 << 16), 256, 0, s>>>(M, K, 256, A, lda,
+                                                                                                   0, mx, PA);
+        split_f16_blk_kernel<<<(unsigned)std::min<long long>((rb + 255) / 256, 1 << 16), 256, 0, s>>>(N, K, 128, B, ldb,
+                                                                                                   1, mx + 1, PB);
+    } else {
+        const long long sa = (long long)KB * M * 16, sb = (long long)KB * N * 16;
+        split_f16_kernel<<<(unsigned)std::min<long long>((sa + 255) / 256, 1 << 16), 256, 0, s>>>(M, K, A, lda, 0,
+                                                                                                 mx, PA);
+        split_f16_kernel<<<(unsigned)std::min<long long>((sb + 255) / 256, 1 << 16), 256, 0, s>>>(N, K, B, ldb, 1,
+                                                                                                 mx + 1, PB);
+    }
+    const int slot = r3 ? PROF_GEMM_F16X3_WIDE : PROF_GEMM_F16X3;
+    prof().begin(slot, s);
+    if (r3) {
+        const long long tiles = (long long)((M + 255) / 256) * ((N + 127) / 128);
+        require(tiles * 512 < (1LL << 32), "split-FP16 GEMM: too many tiles for one dispatch");
+        cgemm_f16x3_r3_kernel<true><<<(unsigned)tiles, 512, 0, s>>>(M, N, K, (float)alpha, (float)beta, PA, PB, mx,
+                                                                    C, ldc);
+    } else {
+        const long long tiles = (long long)((M + 127) / 128) * ((N + 127) / 128);
+        require(tiles * 256 < (1LL << 32), "split-FP16 GEMM: too many tiles for one dispatch");
+        cgemm_f16x3_kernel<2, 2, true><<<(unsigned)tiles, 256, 0, s>>>(M, N, K, (float)alpha, (float)beta, PA, PB, mx,
+                                                                       C, ldc);
+    }
     // algorithmic (complex-equivalent) flops 8MNK; the kernel issues 3x that on
     // the FP16 matrix cores. bytes: split A and B planes once, C read + written
-    prof().end(PROF_GEMM_F16X3, s, 8.0 * M * N * K, 8.0 * ((double)M * K + (double)K * N) + 16.0 * M * N);
+    prof().end(slot, s, 8.0 * M * N * K, 8.0 * ((double)M * K + (double)K * N) + 16.0 * M * N);
     SIE_HIP(hipGetLastError());
 }
 
@@ -1325,8 +1499,9 @@ __global__ void copy_vec_kernel(int n, const cx<T>* __restrict__ in, cx<T>* __re
     if (i < n) out[i] = in[i];
 }
 
-// Diagonal block size of the single-RHS sweeps: fewer, wider dependent steps
-// for FP32 (the sweeps are bound by the latency of their ~2n/B launches).
+// Diagonal block size of the single-RHS sweeps, by factor type: fewer, wider
+// dependent steps for FP32 (the sweeps are bound by the latency of their ~2n/B
+// launches).
 template <typename T>
 struct SweepCfg {
     static constexpr int B = std::is_same<T, float>::value ? 128 : 64;
@@ -1350,10 +1525,10 @@ static void note_factorisation(const void* A) {
 // workspace), so it is captured once into a HIP graph and replayed:
 // iterative refinement calls it with the same pointers every sweep.
 // invL / invU: diagonal-block inverses for op N, their conjugate transposes for op C.
-template <typename T>
-static void record_sweeps(int op, int n, const cx<T>* LU, long long lda, cx<T>* b, const cx<T>* invL,
+template <typename TF, typename T>
+static void record_sweeps(int op, int n, const cx<TF>* LU, long long lda, cx<T>* b, const cx<T>* invL,
                           const cx<T>* invU, cx<T>* y, cx<T>* t, const int* perm, hipStream_t s) {
-    constexpr int B = SweepCfg<T>::B;
+    constexpr int B = SweepCfg<TF>::B;
     const int nb = (n + B - 1) / B;
     const unsigned g = (unsigned)((n + 255) / 256);
     auto width = [&](int k) { return std::min(B, n - k); };
@@ -1362,7 +1537,7 @@ static void record_sweeps(int op, int n, const cx<T>* LU, long long lda, cx<T>* 
         trsv_diag_kernel<T, B><<<1, 256, 0, s>>>(0, width(0), inv, sop, vec, out);
         for (int k = 0; k + B < n; k += B) {
             const int rows = n - (k + B);
-            trsv_update_kernel<T, B><<<(unsigned)((rows + B - 1) / B), 256, 0, s>>>(
+            trsv_update_kernel<TF, T, B><<<(unsigned)((rows + B - 1) / B), 256, 0, s>>>(
                 width(k), k, LU, lda, out, sop, vec, k + B, n, k + B, width(k + B),
                 inv + (size_t)((k + B) / B) * B * B, out);
         }
@@ -1372,7 +1547,7 @@ static void record_sweeps(int op, int n, const cx<T>* LU, long long lda, cx<T>* 
         const int kl = (nb - 1) * B;
         trsv_diag_kernel<T, B><<<1, 256, 0, s>>>(kl, width(kl), inv + (size_t)(nb - 1) * B * B, sop, vec, out);
         for (int k = kl; k > 0; k -= B) {
-            trsv_update_kernel<T, B><<<(unsigned)(k / B), 256, 0, s>>>(
+            trsv_update_kernel<TF, T, B><<<(unsigned)(k / B), 256, 0, s>>>(
                 width(k), k, LU, lda, out, sop, vec, 0, k, k - B, B, inv + (size_t)((k - B) / B) * B * B, out);
         }
     };
@@ -1417,9 +1592,11 @@ static SweepGraphCache& sweep_graphs() {
     return c;
 }
 
-template <typename T>
-void getrs_vec(int op, int n, const cx<T>* LU, long long lda, const int* ipiv, cx<T>* b, hipStream_t s) {
-    constexpr int B = SweepCfg<T>::B;
+// Factors in TF, right-hand side, inverses and arithmetic in T.
+template <typename TF, typename T>
+void getrs_vec(int op, int n, const cx<TF>* LU, long long lda, const int* ipiv, cx<T>* b, hipStream_t s) {
+    constexpr int B = SweepCfg<TF>::B;
+    constexpr int tkey = (int)(sizeof(TF) * 100 + sizeof(T));  // cache key of the type pair
     const int nb = (n + B - 1) / B;
     const size_t invsz = (size_t)nb * B * B * sizeof(cx<T>);
     unsigned char* w = (unsigned char*)ws(5).get(4 * invsz + 2 * (size_t)n * sizeof(cx<T>) + (size_t)n * sizeof(int));
@@ -1443,9 +1620,9 @@ void getrs_vec(int op, int n, const cx<T>* LU, long long lda, const int* ipiv, c
     const auto it = factor_generation().find(LU);
     const unsigned long long gen = it == factor_generation().end() ? 0 : it->second;
     if (!(gen != 0 && cache.inv_LU == LU && cache.inv_n == n && cache.inv_lda == lda &&
-          cache.inv_tsize == (int)sizeof(T) && cache.inv_gen == gen && cache.inv_w == w)) {
-        diag_inv_kernel<T, B><<<nb, B, 0, s>>>(n, LU, lda, 0, invL);
-        diag_inv_kernel<T, B><<<nb, B, 0, s>>>(n, LU, lda, 1, invU);
+          cache.inv_tsize == tkey && cache.inv_gen == gen && cache.inv_w == w)) {
+        diag_inv_kernel<TF, T, B><<<nb, B, 0, s>>>(n, LU, lda, 0, invL);
+        diag_inv_kernel<TF, T, B><<<nb, B, 0, s>>>(n, LU, lda, 1, invU);
         const long long tot = (long long)nb * B * B;
         const unsigned gt = (unsigned)std::min<long long>((tot + 255) / 256, 1 << 16);
         conj_transpose_blocks_kernel<T, B><<<gt, 256, 0, s>>>(tot, invL, invLh);
@@ -1454,19 +1631,19 @@ void getrs_vec(int op, int n, const cx<T>* LU, long long lda, const int* ipiv, c
         cache.inv_LU = LU;
         cache.inv_n = n;
         cache.inv_lda = lda;
-        cache.inv_tsize = (int)sizeof(T);
+        cache.inv_tsize = tkey;
         cache.inv_gen = gen;
         cache.inv_w = w;
     }
     const cx<T>* iL = op == 0 ? invL : invLh;
     const cx<T>* iU = op == 0 ? invU : invUh;
     if (nb < 8) {  // short sequences: launch directly
-        record_sweeps<T>(op, n, LU, lda, b, iL, iU, y, t, perm, s);
+        record_sweeps<TF, T>(op, n, LU, lda, b, iL, iU, y, t, perm, s);
         return;
     }
     hipGraphExec_t exec = nullptr;
     for (auto& e : cache.entries)
-        if (e.tsize == (int)sizeof(T) && e.op == op && e.n == n && e.LU == LU && e.lda == lda && e.b == b && e.w == w)
+        if (e.tsize == tkey && e.op == op && e.n == n && e.LU == LU && e.lda == lda && e.b == b && e.w == w)
             exec = e.exec;
     if (!exec) {
         // drop graphs that point into an older workspace, keep at most 16
@@ -1482,13 +1659,21 @@ void getrs_vec(int op, int n, const cx<T>* LU, long long lda, const int* ipiv, c
         if (!cache.capture) SIE_HIP(hipStreamCreateWithFlags(&cache.capture, hipStreamNonBlocking));
         hipGraph_t graph = nullptr;
         SIE_HIP(hipStreamBeginCapture(cache.capture, hipStreamCaptureModeThreadLocal));
-        record_sweeps<T>(op, n, LU, lda, b, iL, iU, y, t, perm, cache.capture);
+        record_sweeps<TF, T>(op, n, LU, lda, b, iL, iU, y, t, perm, cache.capture);
         SIE_HIP(hipStreamEndCapture(cache.capture, &graph));
         SIE_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
         SIE_HIP(hipGraphDestroy(graph));
-        cache.entries.push_back(SweepGraph{(int)sizeof(T), op, n, LU, lda, b, w, exec});
+        cache.entries.push_back(SweepGraph{tkey, op, n, LU, lda, b, w, exec});
     }
     SIE_HIP(hipGraphLaunch(exec, s));
+}
+
+// One right-hand side in complex double through complex float LU factors
+// (op 0 = N, 2 = C): the FP32 factors applied in FP64 arithmetic, a fixed
+// linear operator whatever the vector.
+void lu_solve_f32_f64(int op, int n, const float* LU, long long lda, const int* ipiv, double* b, hipStream_t s) {
+    require(op == 0 || op == 2, "op must be 0 (N) or 2 (C)");
+    getrs_vec<float, double>(op, n, (const c32*)LU, lda, ipiv, (c64*)b, s);
 }
 
 // B = conj(B) on an n x nrhs block.
@@ -1520,7 +1705,7 @@ void getrs(int op, int n, int nrhs, const cx<T>* LU, long long lda, const int* i
         return;
     }
     if (nrhs <= 4) {
-        for (int c = 0; c < nrhs; ++c) getrs_vec<T>(op, n, LU, lda, ipiv, B + (long long)c * ldb, s);
+        for (int c = 0; c < nrhs; ++c) getrs_vec<T, T>(op, n, LU, lda, ipiv, B + (long long)c * ldb, s);
         return;
     }
     if (op == 0) {

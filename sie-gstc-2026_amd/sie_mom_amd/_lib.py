@@ -94,6 +94,7 @@ _sig("sie_lu_solve", I, [I, I, I, I, P, LL, P, P, LL, P])
 _sig("sie_zgemv", I, [I, I, I, D, P, LL, P, D, P, P])
 _sig("sie_convert", I, [I, I, LL, LL, P, LL, P, LL, D, P])
 _sig("sie_ir_solve", I, [I, I, I, P, LL, P, LL, P, P, LL, P, LL, D, I, P, P, P])
+_sig("sie_fgmres_ir_solve", I, [I, I, I, P, LL, P, LL, P, P, LL, P, LL, D, I, I, P, P, P])
 _sig("sie_norm2", D, [LL, P, P])
 MATVEC_FN = ctypes.CFUNCTYPE(I, P, P, P, P)
 _sig("sie_gmres", I, [I, I, I, D, MATVEC_FN, P, P, P, ctypes.POINTER(D), ctypes.POINTER(I), ctypes.POINTER(I), P, I,

@@ -118,12 +118,12 @@ class GstcProblem:
     """Mesh, options, excitation and cone objective of one GSTC synthesis step."""
 
     def __init__(self, mesh, options, waves, kind, half_angle_deg=10.0, n_theta=8, n_phi=16, mode="mixed",
-                 ir_tol=1e-12, ir_maxit=30, variant="4m"):
+                 ir_tol=1e-12, ir_maxit=30, variant="4m", ir_method="fgmres"):
         self.mesh, self.options, self.waves, self.kind = mesh, options, waves, kind
         self.dirs, self.weights = cone_directions(half_angle_deg, n_theta, n_phi)
         self.w_dev = torch.from_numpy(self.weights).to("cuda", torch.float64)
         self.mode = mode
-        self.ir_tol, self.ir_maxit = ir_tol, ir_maxit
+        self.ir_tol, self.ir_maxit, self.ir_method = ir_tol, ir_maxit, ir_method
         self.variant = variant
         self.eta0 = physical_parameters(options).eta0
         self.timings = {}
@@ -155,7 +155,8 @@ class GstcProblem:
             lu.factorize(LU32)
             self._sync_time("factor", t0)
             t0 = time.perf_counter()
-            xs, rr, its = ir_solve(Z, LU32, lu.pivots, b, op="N", tol=self.ir_tol, maxit=self.ir_maxit)
+            xs, rr, its = ir_solve(Z, LU32, lu.pivots, b, op="N", tol=self.ir_tol, maxit=self.ir_maxit,
+                                   method=self.ir_method)
             st = SolveState(xs=xs, lu=lu, Z=Z, rec=rec, ir_residual=float(rr[0]), ir_iterations=int(its[0]))
         self._sync_time("solve", t0)
         return st
@@ -174,7 +175,8 @@ class GstcProblem:
             lam = g.clone()
             st.lu.solve(lam, op="C")
         else:
-            lam, _, _ = ir_solve(st.Z, st.lu.factors, st.lu.pivots, g, op="C", tol=self.ir_tol, maxit=self.ir_maxit)
+            lam, _, _ = ir_solve(st.Z, st.lu.factors, st.lu.pivots, g, op="C", tol=self.ir_tol, maxit=self.ir_maxit,
+                                 method=self.ir_method)
         self._sync_time("adjoint_solve", t0)
         t0 = time.perf_counter()
         G = mass_sensitivity(self.mesh, self.options, lam, st.xs)

@@ -104,18 +104,28 @@ def to_precision(A, precision):
     return B
 
 
-def ir_solve(Z, LU32, pivots, B, op="N", tol=1e-12, maxit=30):
-    """FP64 iterative refinement with an FP32 LU. Returns (X, rel_residuals, iterations)."""
+IR_METHODS = ("ir", "fgmres")
+
+
+def ir_solve(Z, LU32, pivots, B, op="N", tol=1e-12, maxit=30, method="ir", restart=20):
+    """FP64-accurate solve with an FP32 LU: classic iterative refinement
+    (method "ir") or flexible GMRES(restart) preconditioned by the FP32 LU
+    (method "fgmres"). Returns (X, rel_residuals, iterations)."""
+    if method not in IR_METHODS:
+        raise ValueError(f"unknown refinement method {method!r}")
     n = Z.shape[0]
     Bm = B if B.dim() == 2 else B.view(n, 1)
     nrhs = Bm.shape[1]
     X = colmajor(n, nrhs)
     rr = np.zeros(nrhs)
     its = np.zeros(nrhs, np.int32)
-    _lib.check(_lib.lib.sie_ir_solve(OPS[op], n, nrhs, _dptr(Z), _ld(Z), _dptr(LU32), _ld(LU32), _dptr(pivots),
-                                     _dptr(Bm), Bm.stride(1) if nrhs > 1 else n, _dptr(X), X.stride(1) if nrhs > 1 else n,
-                                     float(tol), int(maxit), rr.ctypes.data_as(ctypes.c_void_p),
-                                     its.ctypes.data_as(ctypes.c_void_p), _stream()))
+    head = (OPS[op], n, nrhs, _dptr(Z), _ld(Z), _dptr(LU32), _ld(LU32), _dptr(pivots),
+            _dptr(Bm), Bm.stride(1) if nrhs > 1 else n, _dptr(X), X.stride(1) if nrhs > 1 else n, float(tol), int(maxit))
+    tail = (rr.ctypes.data_as(ctypes.c_void_p), its.ctypes.data_as(ctypes.c_void_p), _stream())
+    if method == "ir":
+        _lib.check(_lib.lib.sie_ir_solve(*head, *tail))
+    else:
+        _lib.check(_lib.lib.sie_fgmres_ir_solve(*head, int(restart), *tail))
     return (X if B.dim() == 2 else X[:, 0]), rr, its
 
 

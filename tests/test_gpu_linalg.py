@@ -117,6 +117,52 @@ def test_ir_solve(sm, op, variant):
     assert np.linalg.norm(X.cpu().numpy() - xr) / np.linalg.norm(xr) < 1e-12
 
 
+@pytest.mark.parametrize("op", ["N", "C"])
+def test_fgmres_ir_solve(sm, op):
+    """FGMRES-IR on a system with condition number 1e7 and b = op(A) x0, where
+    the FP32 LU leaves an error factor of ~0.3 per refinement sweep (about 26
+    sweeps to 1e-13): both methods reach 1e-13, the Krylov one in fewer steps,
+    and both solutions have that residual in FP64 on the host."""
+    rng = np.random.default_rng(10)
+    n = 600
+    U, _ = np.linalg.qr(crandn(rng, n, n))
+    V, _ = np.linalg.qr(crandn(rng, n, n))
+    A = (U * np.logspace(0, -7, n)) @ V.conj().T
+    M = A if op == "N" else A.conj().T
+    b = M @ crandn(rng, n, 1)
+    Zd = to_dev(A)
+    LU32 = sm.linalg.to_precision(Zd, 32)
+    lu = sm.linalg.DenseLuSolver(variant="f16x3")
+    lu.factorize(LU32)
+    Xi, ri, iti = sm.linalg.ir_solve(Zd, LU32, lu.pivots, to_dev(b), op=op, tol=1e-13, maxit=80)
+    Xf, rf, itf = sm.linalg.ir_solve(Zd, LU32, lu.pivots, to_dev(b), op=op, tol=1e-13, maxit=80, method="fgmres")
+    print(f"op={op}: IR {iti[0]} sweeps ({ri[0]:.1e}), FGMRES-IR {itf[0]} steps ({rf[0]:.1e})", flush=True)
+    assert ri[0] < 1e-13 and rf[0] < 1e-13
+    assert itf[0] < iti[0]
+    for X in (Xi, Xf):
+        x = X.cpu().numpy()
+        assert np.linalg.norm(M @ x - b) / np.linalg.norm(b) < 1e-13
+
+
+def test_fgmres_ir_restarts(sm):
+    """A restart length of 2 forces several cycles (each ending with the true
+    residual) and still converges."""
+    rng = np.random.default_rng(11)
+    n = 300
+    A = crandn(rng, n, n) + 3 * np.sqrt(n) * np.eye(n)
+    b = crandn(rng, n, 1)
+    Zd = to_dev(A)
+    LU32 = sm.linalg.to_precision(Zd, 32)
+    lu = sm.linalg.DenseLuSolver()
+    lu.factorize(LU32)
+    X, rr, its = sm.linalg.ir_solve(Zd, LU32, lu.pivots, to_dev(b), tol=1e-14, maxit=40, method="fgmres", restart=2)
+    assert rr[0] < 1e-14
+    xr = np.linalg.solve(A, b)
+    assert np.linalg.norm(X.cpu().numpy() - xr) / np.linalg.norm(xr) < 1e-13
+    with pytest.raises(ValueError):
+        sm.linalg.ir_solve(Zd, LU32, lu.pivots, to_dev(b), method="cg")
+
+
 def test_convert_grid_stride(sm):
     # more elements than one capped dispatch covers: exercises the grid-stride loop
     n = 5000
