@@ -1064,17 +1064,23 @@ __global__ void diag_inv_kernel(int n, const cx<TF>* __restrict__ LU, long long 
 // and the workgroup whose rows start at nxt0 (the next diagonal block, nw
 // rows) then solves it: out[nxt0 + i] = sum_j M(i, j) vec[nxt0 + j] with M the
 // block's precomputed inverse (op N) or its conjugate transpose (op C).
+// Threads per workgroup of the sweep kernels: the diagonal-block solve and
+// the update of the next block are each done by one workgroup on the sweep's
+// critical path, so a workgroup keeps B^2 / 1024 independent loads per
+// thread in flight at once.
+constexpr int kSweepThreads = 1024;
+
 template <typename T, int B>
 __device__ __forceinline__ void trsv_block_solve(const cx<T>* __restrict__ M, const cx<T>* v, int nw, T (*spr)[B],
                                                  T (*spi)[B], cx<T>* __restrict__ out) {
     // out[i] = sum_j M[j B + i] v_j: M is the inverse (op N) or its conjugate
     // transpose (op C), column-major, so each load is B consecutive entries
-    constexpr int P = 256 / B, J = B / P;
+    constexpr int P = kSweepThreads / B, J = B / P;
     const int t = threadIdx.x, i = t % B, part = t / B;
     T ar = 0, ai = 0;
     // no early exit: every load is independent and in bounds (M is a full
     // B x B block), so the loop pipelines; entries past nw contribute zero
-#pragma unroll 16
+#pragma unroll 8
     for (int jj = 0; jj < J; ++jj) {
         const int j = part * J + jj;
         const cx<T> x = j < nw ? v[j] : cx<T>{0, 0};
@@ -1105,14 +1111,15 @@ __global__ void conj_transpose_blocks_kernel(long long total, const cx<T>* __res
 }
 
 template <typename TF, typename T, int B>
-__global__ __launch_bounds__(256) void trsv_update_kernel(int w, int k0, const cx<TF>* __restrict__ LU, long long lda,
-                                                          const cx<T>* __restrict__ Y, int op, cx<T>* __restrict__ vec,
-                                                          int r0, int r1, int nxt0, int nw,
-                                                          const cx<T>* __restrict__ invn, cx<T>* __restrict__ out) {
-    constexpr int P = 256 / B;        // op N: column groups
-    constexpr int CG = B / P;         // columns per group
-    constexpr int L = B / 4;          // op C: lanes per row, 4 entries each
-    constexpr int RP = 256 / L;       // op C: rows per pass
+__global__ __launch_bounds__(kSweepThreads) void trsv_update_kernel(int w, int k0, const cx<TF>* __restrict__ LU,
+                                                                    long long lda, const cx<T>* __restrict__ Y, int op,
+                                                                    cx<T>* __restrict__ vec, int r0, int r1, int nxt0,
+                                                                    int nw, const cx<T>* __restrict__ invn,
+                                                                    cx<T>* __restrict__ out) {
+    constexpr int P = kSweepThreads / B;   // op N: column groups
+    constexpr int CG = B / P;              // columns per group
+    constexpr int L = B / 4;               // op C: lanes per row, 4 entries each
+    constexpr int RP = kSweepThreads / L;  // op C: rows per pass
     __shared__ cx<T> sy[B];
     __shared__ cx<T> sres[B];
     __shared__ T spr[P][B], spi[P][B];
@@ -1125,7 +1132,7 @@ __global__ __launch_bounds__(256) void trsv_update_kernel(int w, int k0, const c
         T ar = 0, ai = 0;
         if (r < r1) {
             const cx<TF>* a = LU + (long long)(k0 + grp * CG) * lda + r;
-#pragma unroll 8
+#pragma unroll
             for (int cc = 0; cc < CG; ++cc) {
                 const int c = grp * CG + cc;
                 if (c < w) {
@@ -1154,7 +1161,7 @@ __global__ __launch_bounds__(256) void trsv_update_kernel(int w, int k0, const c
         if (t < B) sres[t] = rb + t < r1 ? vec[rb + t] : cx<T>{0, 0};
         __syncthreads();
         const int sub = t % L, rr = t / L;
-#pragma unroll 4
+#pragma unroll
         for (int pass = 0; pass < B / RP; ++pass) {
             const int row = pass * RP + rr, r = rb + row;
             T ar = 0, ai = 0;
@@ -1190,9 +1197,10 @@ __global__ __launch_bounds__(256) void trsv_update_kernel(int w, int k0, const c
 
 // out[k0 + i] = M vec[k0 + i] for one diagonal block (the first of a sweep).
 template <typename T, int B>
-__global__ __launch_bounds__(256) void trsv_diag_kernel(int k0, int w, const cx<T>* __restrict__ inv, int op,
-                                                        const cx<T>* __restrict__ vec, cx<T>* __restrict__ out) {
-    __shared__ T spr[256 / B][B], spi[256 / B][B];
+__global__ __launch_bounds__(kSweepThreads) void trsv_diag_kernel(int k0, int w, const cx<T>* __restrict__ inv, int op,
+                                                                  const cx<T>* __restrict__ vec,
+                                                                  cx<T>* __restrict__ out) {
+    __shared__ T spr[kSweepThreads / B][B], spi[kSweepThreads / B][B];
     trsv_block_solve<T, B>(inv, vec + k0, w, spr, spi, out + k0);
 }
 
@@ -1534,10 +1542,10 @@ static void record_sweeps(int op, int n, const cx<TF>* LU, long long lda, cx<T>*
     auto width = [&](int k) { return std::min(B, n - k); };
     // forward sweep over blocks 0, B, ...: solve block k, update rows [k + B, n)
     auto forward = [&](const cx<T>* inv, int sop, cx<T>* vec, cx<T>* out) {
-        trsv_diag_kernel<T, B><<<1, 256, 0, s>>>(0, width(0), inv, sop, vec, out);
+        trsv_diag_kernel<T, B><<<1, kSweepThreads, 0, s>>>(0, width(0), inv, sop, vec, out);
         for (int k = 0; k + B < n; k += B) {
             const int rows = n - (k + B);
-            trsv_update_kernel<TF, T, B><<<(unsigned)((rows + B - 1) / B), 256, 0, s>>>(
+            trsv_update_kernel<TF, T, B><<<(unsigned)((rows + B - 1) / B), kSweepThreads, 0, s>>>(
                 width(k), k, LU, lda, out, sop, vec, k + B, n, k + B, width(k + B),
                 inv + (size_t)((k + B) / B) * B * B, out);
         }
@@ -1545,9 +1553,10 @@ static void record_sweeps(int op, int n, const cx<TF>* LU, long long lda, cx<T>*
     // backward sweep over blocks (nb-1)B, ..., 0: solve block k, update rows [0, k)
     auto backward = [&](const cx<T>* inv, int sop, cx<T>* vec, cx<T>* out) {
         const int kl = (nb - 1) * B;
-        trsv_diag_kernel<T, B><<<1, 256, 0, s>>>(kl, width(kl), inv + (size_t)(nb - 1) * B * B, sop, vec, out);
+        trsv_diag_kernel<T, B><<<1, kSweepThreads, 0, s>>>(kl, width(kl), inv + (size_t)(nb - 1) * B * B, sop, vec,
+                                                            out);
         for (int k = kl; k > 0; k -= B) {
-            trsv_update_kernel<TF, T, B><<<(unsigned)(k / B), 256, 0, s>>>(
+            trsv_update_kernel<TF, T, B><<<(unsigned)(k / B), kSweepThreads, 0, s>>>(
                 width(k), k, LU, lda, out, sop, vec, 0, k, k - B, B, inv + (size_t)((k - B) / B) * B * B, out);
         }
     };
