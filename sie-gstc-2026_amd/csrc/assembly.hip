@@ -96,7 +96,12 @@ __global__ void near_list_kernel(int nf, int n0, int nc, const double* __restric
 // ------------------------------------------------------------------ step 2
 // W layout: W[((nl * NS) + s) * nf + m], s = (kq*2 + op)*9 + j*3 + i,
 // op 0 = TL, 1 = TK; j = test local edge, i = source local edge.
-template <int NK>
+// SYM (one chunk holding every source face, n0 = 0): only pairs with test
+// face m > source face n are integrated, and each also fills the slot of the
+// swapped pair (test n, source m) with the transposed blocks, which are equal:
+// TL(n, m)[i][j] = TL(m, n)[j][i] because the same rule integrates both faces,
+// and TK likewise since swapping the faces flips both rho_j x rho_i and r - r'.
+template <int NK, bool SYM>
 __global__ __launch_bounds__(256) void far_pairs_kernel(
     int nf, int n0, int nc, int Q, const double* __restrict__ c, const double* __restrict__ o,
     const double* __restrict__ len, const double* __restrict__ bq, const float* __restrict__ qxy,
@@ -106,8 +111,9 @@ __global__ __launch_bounds__(256) void far_pairs_kernel(
     const int nl = uniform(blockIdx.y * 4 + (threadIdx.x >> 6));
     if (nl >= nc) return;
     const int n = n0 + nl;
+    if (SYM && (int)blockIdx.x * 64 + 63 <= n) return;  // no test face of this wave lies above n
     const int mm = m < nf ? m : nf - 1;
-    const bool active = (m < nf) && !is_near(c, nf, mm, n, thr);
+    const bool active = (m < nf) && (!SYM || m > n) && !is_near(c, nf, mm, n, thr);
 
     // test-face geometry (per lane)
     double om[3][3];
@@ -246,6 +252,11 @@ __global__ __launch_bounds__(256) void far_pairs_kernel(
                 tk.im *= 0.25 * sc;
                 Wn[(size_t)((w * 2 + 0) * 9 + j * 3 + i) * nf] = tl;
                 Wn[(size_t)((w * 2 + 1) * 9 + j * 3 + i) * nf] = tk;
+                if (SYM) {
+                    cplx* Wt = W + (size_t)m * NS * nf + n;
+                    Wt[(size_t)((w * 2 + 0) * 9 + i * 3 + j) * nf] = tl;
+                    Wt[(size_t)((w * 2 + 1) * 9 + i * 3 + j) * nf] = tk;
+                }
             }
     }
 }
@@ -871,7 +882,8 @@ struct Assembler {
 
     // Runs steps 1-3 for source faces [n0, n0+nc) into W.
     template <int NK>
-    void chunk_blocks(int n0, int nc, double k1, double k2, float thr, DevBuf<double>& area, hipStream_t s) {
+    void chunk_blocks(int n0, int nc, double k1, double k2, float thr, DevBuf<double>& area, hipStream_t s,
+                      bool sym = false) {
         const int nf = mesh->nf;
         const int Q = quad.n;
         // near list
@@ -886,10 +898,18 @@ struct Assembler {
         // far pairs
         dim3 grid((nf + 63) / 64, (nc + 3) / 4);
         prof().begin(PROF_FAR_PAIRS, s);
-        far_pairs_kernel<NK><<<grid, 256, 0, s>>>(nf, n0, nc, Q, mesh->d_c.p, mesh->d_o.p, mesh->d_len.p, bq.p,
-                                                  quad.xy.p, quad.w.p, k1, k2, thr, W.p);
-        // flops of the moment-factored point-pair update (kFarPairFlops per pair and wavenumber)
-        prof().end(PROF_FAR_PAIRS, s, (double)nf * nc * Q * Q * (kFarGeomFlops + NK * kFarPairFlops),
+        if (sym) {
+            require(n0 == 0 && nc == nf, "symmetric far pairs need every source face in one chunk");
+            far_pairs_kernel<NK, true><<<grid, 256, 0, s>>>(nf, n0, nc, Q, mesh->d_c.p, mesh->d_o.p, mesh->d_len.p,
+                                                            bq.p, quad.xy.p, quad.w.p, k1, k2, thr, W.p);
+        } else {
+            far_pairs_kernel<NK, false><<<grid, 256, 0, s>>>(nf, n0, nc, Q, mesh->d_c.p, mesh->d_o.p, mesh->d_len.p,
+                                                             bq.p, quad.xy.p, quad.w.p, k1, k2, thr, W.p);
+        }
+        // flops of the moment-factored point-pair update (kFarPairFlops per pair and wavenumber),
+        // over the pairs integrated (m > n only when symmetric)
+        const double pairs = sym ? 0.5 * (double)nf * (nf - 1) : (double)nf * nc;
+        prof().end(PROF_FAR_PAIRS, s, pairs * Q * Q * (kFarGeomFlops + NK * kFarPairFlops),
                    (double)nf * nc * NK * 18 * 16.0);
         SIE_HIP(hipGetLastError());
         SIE_HIP(hipStreamSynchronize(s));
@@ -907,6 +927,18 @@ struct Assembler {
                                                                    quad.w.p, k1, k2, W.p);
             SIE_HIP(hipGetLastError());
         }
+    }
+
+    // One chunk of every source face, integrated with the pair symmetry, when
+    // its scratch (F^2 NK 18 complex) stays under kSymBytes and fits in free
+    // device memory with a margin; otherwise scratch_bytes-sized chunks.
+    static constexpr size_t kSymBytes = size_t(64) << 30;
+    bool symmetric_fits(int NK) const {
+        const size_t need = (size_t)mesh->nf * mesh->nf * NK * 18 * sizeof(cplx);
+        if (need > kSymBytes) return false;
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) return false;
+        return need + (size_t(8) << 30) <= fr + W.n * sizeof(cplx);
     }
 
     int chunk_faces(int NK) const {
@@ -947,7 +979,10 @@ static void assemble_generic(sie_mesh* h, const sie_calderon_params* p, const Ep
     Mesh& m = h->m;
     Assembler& A = *h->as;
     A.prepare(p->rule, s);
-    const int nc_max = A.chunk_faces(NK);
+    // assembly into Z integrates each far pair once (both orientations);
+    // products keep the chunked form so that the slab stays a few columns wide
+    const bool sym = x == nullptr && A.symmetric_fits(NK);
+    const int nc_max = sym ? m.nf : A.chunk_faces(NK);
     const size_t wsz = (size_t)nc_max * NK * 18 * m.nf;
     if (A.W.n < wsz) A.W.alloc(wsz);
     const int blocks = KIND == K_GSTC ? 4 : (KIND == K_PMCHWT ? 2 : 1);
@@ -962,7 +997,7 @@ static void assemble_generic(sie_mesh* h, const sie_calderon_params* p, const Ep
     list.reserve(m.ne);
     for (int n0 = 0; n0 < m.nf; n0 += nc_max) {
         const int nc = std::min(nc_max, m.nf - n0);
-        A.chunk_blocks<NK>(n0, nc, (double)p->k1, (double)p->k2, p->threshold, h->area, s);
+        A.chunk_blocks<NK>(n0, nc, (double)p->k1, (double)p->k2, p->threshold, h->area, s, sym);
         // edges with a face in the chunk
         list.clear();
         for (int e = 0; e < m.ne; ++e) {
@@ -994,6 +1029,7 @@ static void assemble_generic(sie_mesh* h, const sie_calderon_params* p, const Ep
         SIE_HIP(hipGetLastError());
         SIE_HIP(hipStreamSynchronize(s));  // list buffer reused by the next chunk
     }
+    if (wsz * sizeof(cplx) > A.scratch_bytes) A.W.release();  // the whole-matrix scratch is not kept
 }
 
 }  // namespace sie
