@@ -824,11 +824,23 @@ __global__ void laswp_kernel(cx<T>* __restrict__ A, long long lda, int c0, int n
 //   2: conj-transposed non-unit upper U^H (forward), 3: conj-transposed unit lower L^H (backward).
 // The triangle T lives at A (column-major, lda); B at B (ldb). One lane per
 // column; the column is held in registers (fully unrolled, static indices).
-template <typename T, int WM>
+template <typename T, int WM, int UPLO>
 __global__ __launch_bounds__(256) void trsm_small_kernel(int w, const cx<T>* __restrict__ Tm, long long lda,
-                                                         cx<T>* __restrict__ B, long long ldb, int ncols, int uplo) {
+                                                         cx<T>* __restrict__ B, long long ldb, int ncols) {
+    constexpr int uplo = UPLO;
+    constexpr int TL = WM * WM / 256;  // triangle entries per thread
     __shared__ cx<T> sT[WM * WM];  // column-major w x w
-    for (int t = threadIdx.x; t < w * w; t += blockDim.x) sT[(t / w) * WM + (t % w)] = Tm[(long long)(t / w) * lda + (t % w)];
+    {
+        // the whole triangle in one round trip: every load issued before any LDS store
+        cx<T> v[TL];
+#pragma unroll
+        for (int q = 0; q < TL; ++q) {
+            const int t = q * 256 + threadIdx.x, col = t / WM, row = t % WM;
+            v[q] = (col < w && row < w) ? Tm[(long long)col * lda + row] : cx<T>{0, 0};
+        }
+#pragma unroll
+        for (int q = 0; q < TL; ++q) sT[q * 256 + threadIdx.x] = v[q];
+    }
     __syncthreads();
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncols) return;
@@ -849,13 +861,13 @@ __global__ __launch_bounds__(256) void trsm_small_kernel(int w, const cx<T>* __r
         const T q = d.re * d.re + d.im * d.im;
         return cx<T>{(v.re * d.re + v.im * d.im) / q, (v.im * d.re - v.re * d.im) / q};
     };
-    if (uplo == 0) {
+    if constexpr (uplo == 0) {
 #pragma unroll
         for (int i = 1; i < WM; ++i)
 #pragma unroll
             for (int k = 0; k < i; ++k)
                 if (i < w) msub(x[i], at(i, k), x[k]);
-    } else if (uplo == 1) {
+    } else if constexpr (uplo == 1) {
 #pragma unroll
         for (int i = WM - 1; i >= 0; --i) {
             if (i < w) {
@@ -865,7 +877,7 @@ __global__ __launch_bounds__(256) void trsm_small_kernel(int w, const cx<T>* __r
                 x[i] = divd(x[i], at(i, i));
             }
         }
-    } else if (uplo == 2) {  // U^H lower: (U^H)(i,k) = conj(U(k,i))
+    } else if constexpr (uplo == 2) {  // U^H lower: (U^H)(i,k) = conj(U(k,i))
 #pragma unroll
         for (int i = 0; i < WM; ++i) {
             if (i < w) {
@@ -1345,7 +1357,13 @@ static void trsm_small(int w, const cx<T>* Tm, long long lda, cx<T>* B, long lon
     if (ncols <= 0 || w <= 0) return;
     constexpr int WM = 32;  // panel width (FP32) and the multi-RHS solve block
     require(w <= WM, "internal: triangular block wider than the register tile");
-    trsm_small_kernel<T, WM><<<(ncols + 255) / 256, 256, 0, s>>>(w, Tm, lda, B, ldb, ncols, uplo);
+    const unsigned g = (unsigned)((ncols + 255) / 256);
+    switch (uplo) {
+        case 0: trsm_small_kernel<T, WM, 0><<<g, 256, 0, s>>>(w, Tm, lda, B, ldb, ncols); break;
+        case 1: trsm_small_kernel<T, WM, 1><<<g, 256, 0, s>>>(w, Tm, lda, B, ldb, ncols); break;
+        case 2: trsm_small_kernel<T, WM, 2><<<g, 256, 0, s>>>(w, Tm, lda, B, ldb, ncols); break;
+        default: trsm_small_kernel<T, WM, 3><<<g, 256, 0, s>>>(w, Tm, lda, B, ldb, ncols); break;
+    }
     SIE_HIP(hipGetLastError());
 }
 
