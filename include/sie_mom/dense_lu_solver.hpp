@@ -1,8 +1,8 @@
 // Operator-level dense solvers over libsie_amd: DenseLuSolver and
 // solve_dense_lu with the reference's interface (include/sie_mom/cuda/
 // dense_lu_solver.cuh:17-197, namespace sie_mom::cuda there, sie_mom::gpu here),
-// plus MixedLuSolver: the FP32-accurate matrix-core LU with FP64 iterative
-// refinement. Matrices are caller-owned complex128 device arrays, column-major.
+// plus MixedLuSolver: the FP32-accurate matrix-core LU with FP64 refinement
+// (GMRES-IR). Matrices are caller-owned complex128 device arrays, column-major.
 #pragma once
 
 #include <vector>
@@ -57,7 +57,10 @@ public:
     MixedLuSolver& operator=(const MixedLuSolver&) = delete;
 
     void factorize(const complex_t* matrix, int dimension, long long ld = 0);
-    // Returns the per-column relative residuals reached (<= tol unless maxit ran out).
+    // GMRES-IR: GMRES(20) right-preconditioned by the FP32 factors (applied in
+    // FP64 arithmetic) on the FP64 matrix; maxit bounds the preconditioned steps
+    // per column. Returns the per-column relative residuals reached (<= tol
+    // unless maxit ran out).
     std::vector<double> solve(Op op, int nrhs, const complex_t* rhs, int ldb, complex_t* x, int ldx,
                               double tol = 1e-12, int maxit = 30, std::vector<int>* sweeps = nullptr) const;
     int dimension() const { return n_; }
