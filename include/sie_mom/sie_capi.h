@@ -144,6 +144,19 @@ int sie_rhs_plane_waves(sie_mesh_t mesh, int kind, int rule, const sie_plane_wav
                         int nwaves, float mu0, float eps0, double vac_mu, double vac_eps,
                         double* rhs_dev, long long ldb, void* stream);
 
+/* E and H radiated at nobs observation points (obs_dev: nobs x 3 doubles) by
+ * physical RWG currents J_dev, M_dev (one complex per edge) in a medium
+ * (epsilon, mu absolute; k = omega sqrt(mu epsilon)): E_dev, H_dev receive
+ * nobs x 3 complex. Replaces evaluate_surface_integral_fields
+ * (src/sie_mom/solvers/surface_current_fields.cu:170-299) and its kernel
+ * compute_scattered_fields_kernel (include/sie_mom/cuda/sie_kernels_utils.cuh:
+ * 21-120): E = jw mu L(J) + K(M), H = -K(J) + jw eps L(M), faces within 0.1
+ * lambda of a point integrated with the Wilton singular part. Synchronises
+ * the stream. */
+int sie_surface_fields(sie_mesh_t mesh, int rule, double omega, double epsilon, double mu,
+                       const double* J_dev, const double* M_dev, const double* obs_dev, int nobs,
+                       double* E_dev, double* H_dev, void* stream);
+
 /* ------------------------------------------------------------ dense algebra */
 /* precision: 64 = complex double (interleaved doubles), 32 = complex float.
  * op: 0 = N, 1 = T (transpose), 2 = C (conjugate transpose); the numbering

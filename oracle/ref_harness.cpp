@@ -409,3 +409,53 @@ extern "C" double ref_sample_pairs(const double* V, const int* F, int nf, int m0
     }
     return sum;
 }
+
+// The field loop of compute_scattered_fields_kernel (sie_kernels_utils.cuh:21-120)
+// run sequentially over faces for each observation point, calling the
+// reference's calculateScatterFieldQuadrature / calculateScatterFieldSingular;
+// J, M, E, H interleaved complex doubles (E, H: 3 per point).
+extern "C" void ref_fields(const double* V, const int* F, const int* FE, int nf, int rule, const double* J,
+                           const double* M, const double* obs, int nobs, double omega, double eps, double mu,
+                           double k, double* E, double* H) {
+    Quad q = quad(rule);
+    const complex__float cJ = make_complex(0.0, omega * mu), cM = make_complex(1.0, 0.0);
+    const complex__float cJh = make_complex(-1.0, 0.0), cMh = make_complex(0.0, omega * eps);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int o = 0; o < nobs; ++o) {
+        const vec3_t r = v3(obs + 3 * o);
+        const complex__float z = make_complex(0.0, 0.0);
+        cvec3_t e = make_cvec3(z, z, z), h = make_cvec3(z, z, z);
+        for (int f = 0; f < nf; ++f) {
+            vec3_t t[3]; int idx[3];
+            load_face(V, F, f, t, idx);
+            const vec3_t c = (t[0] + t[1] + t[2]) / 3.f;
+            const float dist = length(r - c);
+            const float lambda = 2.0f * static_cast<float>(M_PI) / k;
+            const float thr = 0.1f * lambda;
+            cvec3_t L[3], K[3];
+            if (dist < thr) {
+                calculateScatterFieldQuadrature(r, t, idx, q.pts.data(), q.w.data(), q.n, k, true, L, K);
+                cvec3_t Ls[3], Ks[3];
+                calculateScatterFieldSingular(r, t, idx, k, Ls, Ks);
+                for (int i = 0; i < 3; ++i) { L[i] += Ls[i]; K[i] += Ks[i]; }
+            } else {
+                calculateScatterFieldQuadrature(r, t, idx, q.pts.data(), q.w.data(), q.n, k, false, L, K);
+            }
+            for (int i = 0; i < 3; ++i) {
+                const int ed = FE[3 * f + i];
+                const complex__float Je = make_complex(J[2 * ed], J[2 * ed + 1]);
+                const complex__float Me = make_complex(M[2 * ed], M[2 * ed + 1]);
+                e += complexMul(cJ, Je) * L[i];
+                e += complexMul(cM, Me) * K[i];
+                h += complexMul(cJh, Je) * K[i];
+                h += complexMul(cMh, Me) * L[i];
+            }
+        }
+        const complex__float* pe = &e.x;
+        const complex__float* ph = &h.x;
+        for (int x = 0; x < 3; ++x) {
+            E[6 * o + 2 * x] = pe[x].x; E[6 * o + 2 * x + 1] = pe[x].y;
+            H[6 * o + 2 * x] = ph[x].x; H[6 * o + 2 * x + 1] = ph[x].y;
+        }
+    }
+}

@@ -728,6 +728,128 @@ void oracle_assemble_gstc(const double* V, const int* F, const int* FE, int nf, 
                             tensors, uniform, basis, with_calderon, with_mass, 1, Z);
 }
 
+// ------------------------------------------------------------ surface fields
+// E and H radiated by physical RWG currents J, M (one complex per edge) at
+// observation points, as evaluate_surface_integral_fields
+// (surface_current_fields.cu:170-299) computes them through
+// compute_scattered_fields_kernel (sie_kernels_utils.cuh:21-120):
+//   E = jw mu sum J_e L_e + sum M_e K_e,   H = -sum J_e K_e + jw eps sum M_e L_e
+// with the per-face L, K vectors of calculateScatterFieldQuadrature
+// (dunavant_quadrature.cuh:1634-1712) and, for faces whose float centroid
+// distance is below 0.1 lambda (float), the regularised Green's function plus
+// calculateScatterFieldSingular (:1719-1766). Faces are summed in index order
+// (the kernel's per-thread strided sums and tree reduction reorder them).
+namespace {
+// greenAndGradSourceGreenFunction / ...Regular (green_functions_utils.cuh:142-210):
+// G and its gradient with respect to the source point, R = r1 - r2.
+void green_grad(V3 r1, V3 r2, double k, bool regular, C& G, C g[3]) {
+    const V3 R = {r1.x - r2.x, r1.y - r2.y, r1.z - r2.z};
+    const double Rm = std::sqrt(R.x * R.x + R.y * R.y + R.z * R.z);
+    if (!regular) {
+        if (Rm < 1e-12) { G = {0, 0}; g[0] = g[1] = g[2] = {0, 0}; return; }
+        const double iR = 1.0 / Rm;
+        const double c4 = 1.0 / (4.0 * kPi * Rm);
+        const double s = std::sin(k * Rm), co = std::cos(k * Rm);
+        G = {co * c4, -s * c4};
+        const C wg = cmul({iR, k}, G);
+        const double Rh[3] = {R.x * iR, R.y * iR, R.z * iR};
+        for (int x = 0; x < 3; ++x) g[x] = cscl(wg, Rh[x]);
+        return;
+    }
+    if (Rm < 1e-6) { G = {0.0, -k / (4.0 * kPi)}; g[0] = g[1] = g[2] = {0, 0}; return; }
+    const double i4 = 1.0 / (4.0 * kPi), iR = 1.0 / Rm, iR2 = iR * iR, iR3 = iR2 * iR, i4R = i4 * iR;
+    const double s = std::sin(k * Rm), co = std::cos(k * Rm);
+    G = {(co - 1.0) * i4R, -s * i4R};
+    C Kr = cmul({iR3, k * iR2}, {co, -s});
+    Kr.re += -(iR3 + k * k * 0.5 * iR);
+    const C sc = cscl(Kr, i4);
+    const double Rv[3] = {R.x, R.y, R.z};
+    for (int x = 0; x < 3; ++x) g[x] = cscl(sc, Rv[x]);
+}
+}  // namespace
+
+void oracle_fields(const double* Vx, const int* Fx, const int* FE, int nf, int rule, const double* J,
+                   const double* Mc, const double* obs, int nobs, double omega, double eps, double mu, double k,
+                   double* E, double* H) {
+    const Rule q = rule_of(rule);
+    const float kf = (float)k;
+    const float ksq_inv = 1.f / (kf * kf);
+    const float lambda = (float)(2.0f * (float)kPi / k);
+    const float thr = 0.1f * lambda;
+    const C cJ = {0.0, omega * mu}, cMh = {0.0, omega * eps};
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int o = 0; o < nobs; ++o) {
+        const V3 r = {obs[3 * o], obs[3 * o + 1], obs[3 * o + 2]};
+        C e[3] = {{0, 0}, {0, 0}, {0, 0}}, h[3] = {{0, 0}, {0, 0}, {0, 0}};
+        for (int f = 0; f < nf; ++f) {
+            V3 t[3]; int idx[3];
+            load(Vx, Fx, f, t, idx);
+            const V3 cs = t[0] + t[1] + t[2];
+            const V3 c = {cs.x / 3.0, cs.y / 3.0, cs.z / 3.0};
+            const float dist = (float)norm(r - c);
+            const bool near = dist < thr;
+            V3 fv[3];
+            float el[3];
+            for (int i = 0; i < 3; ++i) el[i] = (float)edge_info(idx, t, i, fv[i]);
+            C L[3][3] = {}, K[3][3] = {};
+            for (int p = 0; p < q.n; ++p) {
+                const double wq = q.w[p];
+                const V3 rq = map_point(t, q.xy[2 * p], q.xy[2 * p + 1]);
+                C G, g[3];
+                green_grad(r, rq, (double)kf, near, G, g);
+                for (int i = 0; i < 3; ++i) {
+                    const V3 rho = rq - fv[i];
+                    const double rv[3] = {rho.x, rho.y, rho.z};
+                    for (int x = 0; x < 3; ++x) {
+                        const C t1 = cscl(cscl(G, rv[x]), 0.5);
+                        const C t2 = cscl(g[x], -(double)ksq_inv);
+                        L[i][x] = cadd(L[i][x], cscl(cadd(t1, t2), wq));
+                    }
+                    // 0.5 rho x grad G
+                    const int y1[3] = {1, 2, 0}, z1[3] = {2, 0, 1};
+                    for (int x = 0; x < 3; ++x) {
+                        const C cr = csub(cscl(g[z1[x]], rv[y1[x]]), cscl(g[y1[x]], rv[z1[x]]));
+                        K[i][x] = cadd(K[i][x], cscl(cscl(cr, 0.5), wq));
+                    }
+                }
+            }
+            for (int i = 0; i < 3; ++i)
+                for (int x = 0; x < 3; ++x) { L[i][x] = cscl(L[i][x], (double)el[i]); K[i][x] = cscl(K[i][x], (double)el[i]); }
+            if (near) {
+                const float A = (float)(0.5f * norm(cross(t[1] - t[0], t[2] - t[0])));
+                const float base = 1.0f / (8.0f * (float)kPi * A);
+                const Wilton W = wilton(r, t);
+                for (int i = 0; i < 3; ++i) {
+                    const V3 Rf = r - fv[i];
+                    const V3 ps = psi1(W, fv[i]);
+                    const V3 ph3 = W.phi3b + scale(W.phi2, W.rpi - fv[i]);
+                    const V3 Li = (ps - scale(0.5f * W.phi0, Rf)) + scale((double)ksq_inv, ph3 - scale(W.phi2, Rf));
+                    const V3 Ki = upsilon(W, fv[i], r, kf);
+                    const float ls = el[i] * base;
+                    const V3 Lv = scale(2.0f, Li);
+                    const double lv[3] = {Lv.x * ls, Lv.y * ls, Lv.z * ls}, kv[3] = {Ki.x * ls, Ki.y * ls, Ki.z * ls};
+                    for (int x = 0; x < 3; ++x) { L[i][x].re += lv[x]; K[i][x].re += kv[x]; }
+                }
+            }
+            for (int i = 0; i < 3; ++i) {
+                const int ed = FE[3 * f + i];
+                const C Je = {J[2 * ed], J[2 * ed + 1]}, Me = {Mc[2 * ed], Mc[2 * ed + 1]};
+                const C wEJ = cmul(cJ, Je), wEM = Me, wHJ = {-Je.re, -Je.im}, wHM = cmul(cMh, Me);
+                for (int x = 0; x < 3; ++x) {
+                    e[x] = cadd(e[x], cmul(wEJ, L[i][x]));
+                    e[x] = cadd(e[x], cmul(wEM, K[i][x]));
+                    h[x] = cadd(h[x], cmul(wHJ, K[i][x]));
+                    h[x] = cadd(h[x], cmul(wHM, L[i][x]));
+                }
+            }
+        }
+        for (int x = 0; x < 3; ++x) {
+            E[6 * o + 2 * x] = e[x].re; E[6 * o + 2 * x + 1] = e[x].im;
+            H[6 * o + 2 * x] = h[x].re; H[6 * o + 2 * x + 1] = h[x].im;
+        }
+    }
+}
+
 // PEC T-EFIE (sie_kernels_assembly.cuh:610-705; pec_solver.cu:352-359 passes k2 twice).
 void oracle_assemble_pec(const double* V, const int* F, const int* FE, int nf, int ne, int rule,
                          float k2, float mu_r2, float mu0, float omega, float thr, double* Z) {

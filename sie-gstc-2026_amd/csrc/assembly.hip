@@ -476,6 +476,150 @@ __device__ __forceinline__ void zacc(cplx* Z, long long ldz, long long r, long l
     *p = z;
 }
 
+// --------------------------------------------------------- surface fields
+// E and H at observation points from physical RWG currents J, M (one complex
+// per edge): evaluate_surface_integral_fields (surface_current_fields.cu:170-299)
+// through compute_scattered_fields_kernel (sie_kernels_utils.cuh:21-120):
+//   E = jw mu sum J L + sum M K,   H = -sum J K + jw eps sum M L,
+// with L_i = l_i sum_q w_q (rho_i G / 2 - grad'G / k^2) and
+// K_i = l_i sum_q w_q rho_i x grad'G / 2 (calculateScatterFieldQuadrature,
+// dunavant_quadrature.cuh:1634-1712); faces whose float centroid distance is
+// below 0.1 lambda (float) use the regularised Green's function plus the
+// Wilton singular part (calculateScatterFieldSingular, :1719-1766). One
+// workgroup per observation point, its lanes over faces, then a fixed-order
+// tree reduction (deterministic).
+constexpr int kFieldThreads = 256;
+
+__global__ __launch_bounds__(kFieldThreads) void surface_fields_kernel(
+    int nf, int Q, const double* __restrict__ Vx, const int* __restrict__ Fx, const int* __restrict__ edge,
+    const double* __restrict__ c, const double* __restrict__ len, const float* __restrict__ qxy,
+    const float* __restrict__ qw, const cplx* __restrict__ J, const cplx* __restrict__ M,
+    const double* __restrict__ obs, double omega_mu, double omega_eps, double k, cplx* __restrict__ E,
+    cplx* __restrict__ H) {
+    const int o = blockIdx.x;
+    const V3 r = {obs[3 * o], obs[3 * o + 1], obs[3 * o + 2]};
+    const float kf = (float)k;
+    const float ksq = 1.f / (kf * kf);
+    const float lambda = (float)(2.0f * (float)kPi / k);
+    const float thr = 0.1f * lambda;
+    const double kd = (double)kf;
+    cplx e[3] = {{0, 0}, {0, 0}, {0, 0}}, h[3] = {{0, 0}, {0, 0}, {0, 0}};
+    for (int f = threadIdx.x; f < nf; f += kFieldThreads) {
+        V3 t[3];
+        for (int a = 0; a < 3; ++a) {
+            const int v = Fx[3 * f + a];
+            t[a] = {Vx[3 * v], Vx[3 * v + 1], Vx[3 * v + 2]};
+        }
+        const V3 d = {r.x - c[f], r.y - c[nf + f], r.z - c[2 * nf + f]};
+        const bool near = (float)sqrt(vdot(d, d)) < thr;
+        V3 fv[3];
+        float el[3];
+        for (int i = 0; i < 3; ++i) {
+            fv[i] = t[(i + 2) % 3];
+            el[i] = (float)len[i * (size_t)nf + f];
+        }
+        cplx L[3][3], K[3][3];
+        for (int i = 0; i < 3; ++i)
+            for (int x = 0; x < 3; ++x) L[i][x] = K[i][x] = {0, 0};
+        for (int q = 0; q < Q; ++q) {
+            const double xi = qxy[2 * q], eta = qxy[2 * q + 1], wq = qw[q];
+            const double ze = 1.0 - xi - eta;
+            const V3 rq = {t[0].x * ze + t[1].x * xi + t[2].x * eta, t[0].y * ze + t[1].y * xi + t[2].y * eta,
+                           t[0].z * ze + t[1].z * xi + t[2].z * eta};
+            const V3 R = vsub(r, rq);
+            const double Rm = sqrt(vdot(R, R));
+            cplx G = {0, 0}, g[3] = {{0, 0}, {0, 0}, {0, 0}};
+            double sn, cs;
+            sincos_moderate(kd * Rm, &sn, &cs);
+            if (!near) {  // greenAndGradSourceGreenFunction (green_functions_utils.cuh:142-175)
+                if (Rm >= 1e-12) {
+                    const double iR = 1.0 / Rm, c4 = 1.0 / (4.0 * kPi * Rm);
+                    G = {cs * c4, -sn * c4};
+                    const cplx wg = cmul({iR, kd}, G);
+                    const double Rh[3] = {R.x * iR, R.y * iR, R.z * iR};
+                    for (int x = 0; x < 3; ++x) g[x] = {wg.re * Rh[x], wg.im * Rh[x]};
+                }
+            } else if (Rm < 1e-6) {  // ...Regular (:177-222), R -> 0
+                G = {0.0, -kd / (4.0 * kPi)};
+            } else {
+                const double i4 = 1.0 / (4.0 * kPi), iR = 1.0 / Rm, iR2 = iR * iR, iR3 = iR2 * iR, i4R = i4 * iR;
+                G = {(cs - 1.0) * i4R, -sn * i4R};
+                cplx Kr = cmul({iR3, kd * iR2}, {cs, -sn});
+                Kr.re += -(iR3 + kd * kd * 0.5 * iR);
+                const cplx sc = {Kr.re * i4, Kr.im * i4};
+                const double Rv[3] = {R.x, R.y, R.z};
+                for (int x = 0; x < 3; ++x) g[x] = {sc.re * Rv[x], sc.im * Rv[x]};
+            }
+            for (int i = 0; i < 3; ++i) {
+                const double rv[3] = {rq.x - fv[i].x, rq.y - fv[i].y, rq.z - fv[i].z};
+                for (int x = 0; x < 3; ++x) {
+                    const cplx t1 = {G.re * rv[x] * 0.5, G.im * rv[x] * 0.5};
+                    const cplx t2 = {g[x].re * -(double)ksq, g[x].im * -(double)ksq};
+                    L[i][x].re += (t1.re + t2.re) * wq;
+                    L[i][x].im += (t1.im + t2.im) * wq;
+                }
+                for (int x = 0; x < 3; ++x) {
+                    const int y = (x + 1) % 3, z = (x + 2) % 3;
+                    const cplx cr = {g[z].re * rv[y] - g[y].re * rv[z], g[z].im * rv[y] - g[y].im * rv[z]};
+                    K[i][x].re += cr.re * 0.5 * wq;
+                    K[i][x].im += cr.im * 0.5 * wq;
+                }
+            }
+        }
+        for (int i = 0; i < 3; ++i)
+            for (int x = 0; x < 3; ++x) {
+                L[i][x].re *= el[i]; L[i][x].im *= el[i];
+                K[i][x].re *= el[i]; K[i][x].im *= el[i];
+            }
+        if (near) {
+            const V3 cr = vcross(vsub(t[1], t[0]), vsub(t[2], t[0]));
+            const float A = (float)(0.5f * sqrt(vdot(cr, cr)));
+            const float base = 1.0f / (8.0f * (float)kPi * A);
+            const Wil P = wilton(r, t, vunit(cr));
+            const float k2o2 = kf * kf * 0.5f;
+            for (int i = 0; i < 3; ++i) {
+                const V3 Rf = vsub(r, fv[i]);
+                // Psi_1 = phi1_base + phi0 r_pi - phi0 free; phi_3 = phi3_base + (r_pi - free) phi2
+                const V3 ps = vsub(vadd(P.phi1b, vscl(P.rpi, P.phi0)), vscl(fv[i], P.phi0));
+                const V3 ph3 = vadd(P.phi3b, vscl(vsub(P.rpi, fv[i]), P.phi2));
+                const V3 Li = vadd(vsub(ps, vscl(Rf, 0.5 * P.phi0)), vscl(vsub(ph3, vscl(Rf, P.phi2)), (double)ksq));
+                // calculate_Upsilon_coul: -(r - free) x (phi_3 + k^2/2 Psi_1)
+                const V3 Ki = vscl(vcross(Rf, vadd(ph3, vscl(ps, (double)k2o2))), -1.0);
+                const double ls = (double)(el[i] * base);
+                L[i][0].re += 2.0 * Li.x * ls; L[i][1].re += 2.0 * Li.y * ls; L[i][2].re += 2.0 * Li.z * ls;
+                K[i][0].re += Ki.x * ls; K[i][1].re += Ki.y * ls; K[i][2].re += Ki.z * ls;
+            }
+        }
+        for (int i = 0; i < 3; ++i) {
+            const int ed = edge[i * (size_t)nf + f];
+            const cplx Je = J[ed], Me = M[ed];
+            const cplx wEJ = {-omega_mu * Je.im, omega_mu * Je.re};    // j w mu J
+            const cplx wHM = {-omega_eps * Me.im, omega_eps * Me.re};  // j w eps M
+            for (int x = 0; x < 3; ++x) {
+                const cplx a = cmul(wEJ, L[i][x]), b = cmul(Me, K[i][x]);
+                const cplx hc = cmul(Je, K[i][x]), hd = cmul(wHM, L[i][x]);
+                e[x].re += a.re + b.re; e[x].im += a.im + b.im;
+                h[x].re += hd.re - hc.re; h[x].im += hd.im - hc.im;
+            }
+        }
+    }
+    __shared__ double red[12][kFieldThreads];
+    for (int x = 0; x < 3; ++x) {
+        red[2 * x][threadIdx.x] = e[x].re; red[2 * x + 1][threadIdx.x] = e[x].im;
+        red[6 + 2 * x][threadIdx.x] = h[x].re; red[7 + 2 * x][threadIdx.x] = h[x].im;
+    }
+    __syncthreads();
+    for (int st = kFieldThreads / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st)
+            for (int v = 0; v < 12; ++v) red[v][threadIdx.x] += red[v][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) {
+        E[3 * o + threadIdx.x] = {red[2 * threadIdx.x][0], red[2 * threadIdx.x + 1][0]};
+        H[3 * o + threadIdx.x] = {red[6 + 2 * threadIdx.x][0], red[7 + 2 * threadIdx.x][0]};
+    }
+}
+
 // One lane per (test edge em, listed source edge en).
 template <int NK, int KIND>
 __global__ __launch_bounds__(256) void gather_kernel(
@@ -1249,6 +1393,31 @@ int sie_pmchwt_matvec(sie_mesh_t mesh, const sie_calderon_params* p, const doubl
     SIE_GUARD_BEGIN
     require(x_dev != nullptr && y_dev != nullptr && x_dev != y_dev, "matvec needs distinct x and y vectors");
     run_pmchwt(mesh, p, nullptr, 0, (const cplx*)x_dev, (cplx*)y_dev, (hipStream_t)stream);
+    SIE_GUARD_END
+}
+
+int sie_surface_fields(sie_mesh_t mesh, int rule, double omega, double epsilon, double mu, const double* J_dev,
+                       const double* M_dev, const double* obs_dev, int nobs, double* E_dev, double* H_dev,
+                       void* stream) {
+    SIE_GUARD_BEGIN
+    require(mesh != nullptr, "null mesh");
+    require(rule >= 1 && rule <= 20, "Dunavant quadrature rule must be in [1, 20]");
+    require(omega > 0 && epsilon > 0 && mu > 0 && std::isfinite(omega) && std::isfinite(epsilon) &&
+                std::isfinite(mu),
+            "field-evaluation frequency and medium must be positive and finite");
+    require(nobs >= 0, "observation point count must be non-negative");
+    if (nobs == 0) return SIE_OK;
+    require(J_dev && M_dev && obs_dev && E_dev && H_dev, "null field-evaluation argument");
+    hipStream_t s = (hipStream_t)stream;
+    Assembler& A = *mesh->as;
+    quad_upload(A.quad, rule, s);
+    const Mesh& m = mesh->m;
+    const double k = omega * std::sqrt(mu * epsilon);
+    surface_fields_kernel<<<(unsigned)nobs, kFieldThreads, 0, s>>>(
+        m.nf, A.quad.n, m.d_V.p, m.d_F.p, m.d_edge.p, m.d_c.p, m.d_len.p, A.quad.xy.p, A.quad.w.p,
+        (const cplx*)J_dev, (const cplx*)M_dev, obs_dev, omega * mu, omega * epsilon, k, (cplx*)E_dev, (cplx*)H_dev);
+    SIE_HIP(hipGetLastError());
+    SIE_HIP(hipStreamSynchronize(s));
     SIE_GUARD_END
 }
 

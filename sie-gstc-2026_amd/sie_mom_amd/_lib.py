@@ -87,6 +87,7 @@ _sig("sie_assemble_pmchwt", I, [P, ctypes.POINTER(CalderonParams), P, LL, P])
 _sig("sie_gstc_matvec", I, [P, ctypes.POINTER(CalderonParams), P, I, I, I, P, P, P])
 _sig("sie_pec_matvec", I, [P, ctypes.POINTER(CalderonParams), P, P, P])
 _sig("sie_pmchwt_matvec", I, [P, ctypes.POINTER(CalderonParams), P, P, P])
+_sig("sie_surface_fields", I, [P, I, D, D, D, P, P, P, I, P, P, P])
 _sig("sie_rhs_plane_waves", I, [P, I, I, ctypes.POINTER(PlaneWaveC), I, Fl, Fl, D, D, P, LL, P])
 _sig("sie_gemm", I, [I, I, I, I, D, P, LL, P, LL, D, P, LL, I, P])
 _sig("sie_lu_factor", I, [I, I, P, LL, P, ctypes.POINTER(I), I, P])

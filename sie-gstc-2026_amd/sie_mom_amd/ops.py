@@ -147,3 +147,48 @@ def rhs_plane_waves(mesh, options, waves):
         float(options.vacuum_permeability), float(options.vacuum_permittivity),
         _dptr(B), B.stride(1), _stream()))
     return B
+
+
+def surface_fields(mesh, J, M, points, frequency_hz, medium=None, quadrature_rule=7,
+                   vacuum_permittivity=None, vacuum_permeability=None):
+    """E, H (n x 3 complex128 device tensors) radiated at points (n x 3) by the
+    physical per-edge currents J, M: evaluate_surface_integral_fields
+    (surface_current_fields.cu:170-299), with its argument checks."""
+    from .physics import Medium, CODATA2022_EPS0, CODATA2022_MU0
+    medium = medium or Medium(1.0, 1.0)
+    eps0 = CODATA2022_EPS0 if vacuum_permittivity is None else vacuum_permittivity
+    mu0 = CODATA2022_MU0 if vacuum_permeability is None else vacuum_permeability
+    fin = np.isfinite
+    if not (frequency_hz > 0 and fin(frequency_hz)):
+        raise ValueError("field-evaluation frequency must be positive and finite")
+    if not (medium.relative_permittivity > 0 and medium.relative_permeability > 0
+            and fin(medium.relative_permittivity) and fin(medium.relative_permeability)):
+        raise ValueError("field-evaluation relative permittivity and permeability must be positive and finite")
+    if not (eps0 > 0 and mu0 > 0 and fin(eps0) and fin(mu0)):
+        raise ValueError("field-evaluation vacuum constants must be positive and finite")
+    if not 1 <= quadrature_rule <= 20:
+        raise ValueError("Dunavant quadrature rule must be in [1, 20]")
+    pts = torch.as_tensor(points, dtype=torch.float64)
+    if pts.dim() != 2 or pts.shape[1] != 3:
+        raise ValueError("observation points must be an n x 3 array")
+    if not torch.isfinite(pts).all():
+        bad = int((~torch.isfinite(pts).all(dim=1)).nonzero()[0])
+        raise ValueError(f"non-finite observation point at index {bad}")
+    n = pts.shape[0]
+    if J.numel() != mesh.ne or M.numel() != mesh.ne:
+        raise ValueError("field evaluation requires one electric and magnetic coefficient per mesh edge")
+    Ed = torch.empty((n, 3), dtype=torch.complex128, device="cuda")
+    Hd = torch.empty((n, 3), dtype=torch.complex128, device="cuda")
+    if n == 0:
+        return Ed, Hd
+    Jd = J.to(device="cuda", dtype=torch.complex128).contiguous()
+    Md = M.to(device="cuda", dtype=torch.complex128).contiguous()
+    if not (torch.isfinite(torch.view_as_real(Jd)).all() and torch.isfinite(torch.view_as_real(Md)).all()):
+        raise ValueError("non-finite surface-current coefficient")
+    Pd = pts.to("cuda").contiguous()
+    omega = 2.0 * np.pi * frequency_hz
+    eps = medium.relative_permittivity * eps0
+    mu = medium.relative_permeability * mu0
+    _lib.check(_lib.lib.sie_surface_fields(mesh.handle, int(quadrature_rule), omega, eps, mu, _dptr(Jd), _dptr(Md),
+                                           _dptr(Pd), n, _dptr(Ed), _dptr(Hd), _stream()))
+    return Ed, Hd

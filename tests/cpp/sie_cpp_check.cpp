@@ -18,6 +18,7 @@
 
 #include "sie_mom/dense_lu_solver.hpp"
 #include "sie_mom/dielectric_solver.hpp"
+#include "sie_mom/fields.hpp"
 #include "sie_mom/gmres_solver.hpp"
 #include "sie_mom/gstc_sie.hpp"
 #include "sie_mom/io.hpp"
@@ -132,6 +133,28 @@ static int validate(const std::string& dir) {
     dopt.formulation = solvers::DielectricFormulation::N_Muller;
     CHECK(throws<std::invalid_argument>([&] { solvers::solve_dielectric_dense(sphere, dopt, src18); }, "T-PMCHWT"));
     CHECK(throws<std::invalid_argument>([&] { gpu::DenseLuSolver lu(0); }, "dimension must be positive"));
+        // field evaluation: argument checks (host side, before any device work)
+    {
+        using solvers::evaluate_surface_integral_fields;
+        const std::vector<complex_t> cur(sphere.edges.size(), complex_t(1, 0));
+        const std::vector<vec3_t> pts = {make_vec3(0, 0, 2)};
+        CHECK(throws<std::invalid_argument>([&] { evaluate_surface_integral_fields(sphere, cur, cur, pts, 0.0, {}); },
+                                            "field-evaluation frequency must be positive and finite"));
+        IsotropicMedium bad;
+        bad.relative_permittivity = -1;
+        CHECK(throws<std::invalid_argument>([&] { evaluate_surface_integral_fields(sphere, cur, cur, pts, 1e9, bad); },
+                                            "relative permittivity and permeability must be positive"));
+        CHECK(throws<std::invalid_argument>(
+            [&] { evaluate_surface_integral_fields(sphere, cur, cur, pts, 1e9, {}, 21); },
+            "Dunavant quadrature rule must be in [1, 20]"));
+        const std::vector<vec3_t> nanpt = {make_vec3(0, 0, 2), make_vec3(std::nan(""), 0, 0)};
+        CHECK(throws<std::invalid_argument>([&] { evaluate_surface_integral_fields(sphere, cur, cur, nanpt, 1e9, {}); },
+                                            "non-finite observation point at index 1"));
+        CHECK(evaluate_surface_integral_fields(sphere, cur, cur, {}, 1e9, {}).empty());
+        const std::vector<complex_t> shortc(5);
+        CHECK(throws<std::invalid_argument>([&] { evaluate_surface_integral_fields(sphere, shortc, cur, pts, 1e9, {}); },
+                                            "one electric and magnetic coefficient per mesh edge"));
+    }
     std::printf("validate: %d failure(s)\n", failures);
     return failures ? 1 : 0;
 }
@@ -177,6 +200,21 @@ static int solve(const std::string& dir, double f, int rule) {
     po.capture_assembled_matrix = true;
     auto pec = solvers::solve_pec_dense(sphere, po, src18);
     dump(dir + "/pec.bin", pec.exterior_current);
+    {
+        // fields of the PEC currents at far, near-surface and interior points
+        const std::vector<vec3_t> pts = {make_vec3(0, 0, 3), make_vec3(1.5, -0.5, 0.25), make_vec3(0.2, 0.1, -0.3),
+                                         make_vec3(0, 0, 0.6), make_vec3(0.5, 0.5, 0.55)};
+        const std::vector<complex_t> zero(pec.exterior_current.size());
+        const auto smp = solvers::evaluate_surface_integral_fields(sphere, pec.exterior_current, zero, pts, f, {},
+                                                                   rule, po.vacuum_permittivity,
+                                                                   po.vacuum_permeability);
+        std::vector<complex_t> eh;
+        for (const auto& q : smp)
+            for (int c = 0; c < 3; ++c) eh.push_back(q.electric[c]);
+        for (const auto& q : smp)
+            for (int c = 0; c < 3; ++c) eh.push_back(q.magnetic[c]);
+        dump(dir + "/pec_fields.bin", eh);
+    }
     CHECK(pec.assembled_matrix.size() == 120 * 120 && pec.metrics.relative_residual < 1e-12);
     solvers::DielectricOptions dopt;
     dopt.frequency_hz = f;

@@ -9,7 +9,8 @@ per-pair TL/TK blocks (regular and regularised) and singular parts for a
 coincident, an edge-adjacent, a vertex-adjacent and a separated pair; Wilton
 potentials at on-plane, off-plane and edge-line points; the 8-tensor mass
 batch; plane-wave excitation testing; full eta0-scaled GSTC (level 0, both
-bases), PEC T-EFIE (level 1) and T-PMCHWT (level 0) matrices.
+bases), PEC T-EFIE (level 1) and T-PMCHWT (level 0) matrices; E and H of random
+currents on the level-1 sphere at far, near-surface and interior points.
 """
 import os
 import sys
@@ -122,6 +123,16 @@ def main():
     fn(ptr(V), ptr(F), ptr(FE), len(F), len(E), 7, ptr(e0), ptr(kh), kk, et, 2, f32(mu0), f32(eps0), ptr(rhs))
     out["rhs_gstc"] = rhs
     out["rhs_gstc_k_eta"] = np.array([kk, et])
+    # fields of random currents on the level-1 sphere at far, near-surface and interior points
+    rng = np.random.default_rng(21)
+    Jf = rng.normal(size=len(E)) + 1j * rng.normal(size=len(E))
+    Mf = rng.normal(size=len(E)) + 1j * rng.normal(size=len(E))
+    obs = ol.field_points(V, F, rng)
+    omf = 4.0 / np.sqrt(mu0 * eps0)
+    kf = omf * np.sqrt(mu0 * eps0)
+    Ef, Hf = ol.fields(ol.REF_SO, "ref_", (V, F, E, FE), 7, Jf, Mf, obs, omf, eps0, mu0, kf)
+    out.update(fields_J=Jf, fields_M=Mf, fields_obs=obs, fields_phys=np.array([omf, eps0, mu0, kf]),
+               fields_E=Ef, fields_H=Hf)
     np.savez_compressed(os.path.join(HERE, "reference_vectors.npz"), **out)
     print("wrote", len(out), "arrays")
 

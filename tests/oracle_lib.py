@@ -292,3 +292,37 @@ def oracle_farfield_matrix(mesh, rule, k0, eta0, dirs):
             A[:, 0, 3 * ne + e] += coef * eta0 * npp
             A[:, 1, 3 * ne + e] += -coef * eta0 * nt
     return A.reshape(nd * 2, 4 * ne)
+
+
+# ------------------------------------------------------------------ fields
+def fields(so, prefix, mesh, rule, J, M, obs, omega, eps, mu, k):
+    """E, H (nobs x 3 complex) radiated by per-edge currents J, M at points
+    obs (nobs x 3): the oracle (so = ORACLE_SO, prefix "oracle_") or the
+    reference harness (REF_SO, "ref_")."""
+    V, F, E, FE = mesh
+    lib = ctypes.CDLL(so)
+    fn = getattr(lib, prefix + "fields")
+    fn.argtypes = [_P, _P, _P, ctypes.c_int, ctypes.c_int, _P, _P, _P, ctypes.c_int] + [ctypes.c_double] * 4 + [_P, _P]
+    obs = np.ascontiguousarray(obs, float)
+    J = np.ascontiguousarray(J, complex)
+    M = np.ascontiguousarray(M, complex)
+    Eo = np.zeros((len(obs), 3), complex)
+    Ho = np.zeros((len(obs), 3), complex)
+    fn(_ptr(np.ascontiguousarray(V, float)), _ptr(np.ascontiguousarray(F, np.int32)),
+       _ptr(np.ascontiguousarray(FE, np.int32)), len(F), rule, _ptr(J), _ptr(M), _ptr(obs), len(obs),
+       omega, eps, mu, k, _ptr(Eo), _ptr(Ho))
+    return Eo, Ho
+
+
+def field_points(V, F, rng, n_far=6, n_near=6, n_in=4):
+    """Observation points: far (|r| = 2.5), just outside and inside a face
+    (1% of the radius off its centroid), and interior (|r| = 0.4)."""
+    u = rng.normal(size=(n_far, 3))
+    far = 2.5 * u / np.linalg.norm(u, axis=1)[:, None]
+    fs = rng.choice(len(F), n_near, replace=False)
+    c = V[F[fs]].mean(axis=1)
+    nrm = c / np.linalg.norm(c, axis=1)[:, None]
+    near = c + 0.01 * nrm * np.where(np.arange(n_near) % 2 == 0, 1.0, -1.0)[:, None]
+    u = rng.normal(size=(n_in, 3))
+    inner = 0.4 * u / np.linalg.norm(u, axis=1)[:, None]
+    return np.concatenate([far, near, inner])

@@ -45,6 +45,11 @@ def test_cpp_solvers_match_oracle(tmp_path):
     r = subprocess.run([_binary(), "solve", str(tmp_path), repr(f), str(rule)], capture_output=True, text=True,
                        timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
+    # field evaluation of the C++ PEC currents against the oracle
+    eh = _read(tmp_path / "pec_fields.bin").reshape(2, 5, 3)
+    Eo, Ho = _pec_fields_oracle(tmp_path, sm, f, rule, mesh)
+    assert np.abs(eh[0] - Eo).max() < 1e-12 * np.abs(Eo).max()
+    assert np.abs(eh[1] - Ho).max() < 1e-12 * np.abs(Ho).max()
     ne = mesh.ne
 
     def rel(a, b):
@@ -89,3 +94,14 @@ def test_cpp_solvers_match_oracle(tmp_path):
     xd[ne:] *= np.sqrt(do.vacuum_permeability / do.vacuum_permittivity)
     assert rel(_read(tmp_path / "pmchwt.bin"), xd) < 1e-10
     assert "solve: 0 failure(s)" in r.stdout
+
+
+def _pec_fields_oracle(tmp_path, sm, f, rule, mesh):
+    """Oracle fields of the C++ PEC currents at the points the program used."""
+    J = _read(tmp_path / "pec.bin")
+    pts = np.array([[0, 0, 3], [1.5, -0.5, 0.25], [0.2, 0.1, -0.3], [0, 0, 0.6], [0.5, 0.5, 0.55]], float)
+    eps, mu = 8.8541878128e-12, 1.25663706212e-6  # PecOptions defaults (CODATA 2018)
+    om = 2 * np.pi * f
+    k = om * np.sqrt(mu * eps)
+    arrays = (mesh.vertices, mesh.faces, mesh.edges, mesh.face_edges)
+    return ol.fields(ol.ORACLE_SO, "oracle_", arrays, rule, J, np.zeros_like(J), pts, om, eps, mu, k)

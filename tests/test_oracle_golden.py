@@ -103,3 +103,14 @@ def test_rhs():
     mu0, eps0 = 1.25663706127e-6, 8.8541878188e-12
     rhs = ol.oracle_rhs(m1, 0, 7, w, mu0, eps0, mu0, eps0)
     np.testing.assert_array_equal(rhs, G["rhs_gstc"])
+
+
+def test_fields():
+    """E and H of random currents on the level-1 sphere (far, near-surface with
+    singular extraction, interior points) against the reference's numerics."""
+    mesh = (G["ico1_V"], G["ico1_F"], G["ico1_E"], G["ico1_FE"])
+    om, eps, mu, k = G["fields_phys"]
+    Eo, Ho = ol.fields(ol.ORACLE_SO, "oracle_", mesh, 7, G["fields_J"], G["fields_M"], G["fields_obs"], om, eps,
+                       mu, k)
+    np.testing.assert_array_equal(Eo, G["fields_E"])
+    np.testing.assert_array_equal(Ho, G["fields_H"])

@@ -50,3 +50,25 @@ def test_full_gstc_level1():
         Zo = ol.assemble_gstc(ol.ORACLE_SO, "oracle_", (V, F, E, FE), 5, phys, t, False, basis)
         Zr = ol.assemble_gstc(ol.REF_SO, "ref_", (V, F, E, FE), 5, phys, t, False, basis)
         assert np.linalg.norm(Zo - Zr) / np.linalg.norm(Zr) < 1e-15
+
+
+def test_fields_vs_reference():
+    """Surface-current fields: the oracle against the reference's own
+    calculateScatterFieldQuadrature / calculateScatterFieldSingular loop, at
+    far, near-surface (singular extraction) and interior points."""
+    rng = np.random.default_rng(21)
+    mesh = ol.icosphere(1)
+    V, F, E, FE = mesh
+    ne = len(E)
+    J = rng.normal(size=ne) + 1j * rng.normal(size=ne)
+    M = rng.normal(size=ne) + 1j * rng.normal(size=ne)
+    obs = ol.field_points(V, F, rng)
+    eps0, mu0 = 8.8541878188e-12, 1.25663706127e-6
+    for k0, er in ((4.0, 1.0), (2.0, 3.0)):
+        om = k0 / np.sqrt(mu0 * eps0)
+        eps, mu = er * eps0, mu0
+        k = om * np.sqrt(mu * eps)
+        Eo, Ho = ol.fields(ol.ORACLE_SO, "oracle_", mesh, 7, J, M, obs, om, eps, mu, k)
+        Er, Hr = ol.fields(ol.REF_SO, "ref_", mesh, 7, J, M, obs, om, eps, mu, k)
+        np.testing.assert_array_equal(Eo, Er)
+        np.testing.assert_array_equal(Ho, Hr)
