@@ -35,6 +35,13 @@ struct IsotropicMedium {
 
 enum class GstcBasisType : int { face = 0, spherical = 1 };
 
+namespace solvers {
+// Arithmetic of the dense solve: plain FP64 LU, or FP32 LU on the matrix cores
+// with FP64 refinement to refinement_tolerance (an extension of the reference's
+// option structs; fp64 is the reference's behaviour).
+enum class SolvePrecision { fp64, mixed };
+}  // namespace solvers
+
 // CODATA vacuum constants used by the reference's option defaults.
 constexpr real_t kCodata2022Eps0 = 8.8541878188e-12;
 constexpr real_t kCodata2022Mu0 = 1.25663706127e-6;

@@ -58,9 +58,10 @@ public:
 
     void factorize(const complex_t* matrix, int dimension, long long ld = 0);
     // GMRES-IR: GMRES(20) right-preconditioned by the FP32 factors (applied in
-    // FP64 arithmetic) on the FP64 matrix; maxit bounds the preconditioned steps
-    // per column. Returns the per-column relative residuals reached (<= tol
-    // unless maxit ran out).
+    // FP64 arithmetic) on the FP64 matrix, per column; for more than four
+    // columns with op N, blocked refinement sweeps instead. maxit bounds the
+    // steps (sweeps) per column. Returns the per-column relative residuals
+    // reached (<= tol unless maxit ran out).
     std::vector<double> solve(Op op, int nrhs, const complex_t* rhs, int ldb, complex_t* x, int ldx,
                               double tol = 1e-12, int maxit = 30, std::vector<int>* sweeps = nullptr) const;
     int dimension() const { return n_; }

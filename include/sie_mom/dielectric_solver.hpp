@@ -28,6 +28,11 @@ struct DielectricOptions {
     DielectricFormulation formulation = DielectricFormulation::T_PMCHWT;
     NtlIntegralForm ntl_form = NtlIntegralForm::Surface;
     int line_quadrature_points = 20;
+    // extension: FP32 matrix-core LU + FP64 refinement (GMRES-IR for one
+    // right-hand side, blocked refinement sweeps for several)
+    SolvePrecision precision = SolvePrecision::fp64;
+    double refinement_tolerance = 1e-12;
+    int refinement_max_sweeps = 40;
 };
 
 struct DielectricMetrics {
@@ -56,6 +61,19 @@ struct DielectricResult {
 
 DielectricResult solve_dielectric_dense(const mesh::MeshData& mesh, const DielectricOptions& options,
                                         const excitation::PreparedExcitation& source);
+
+// Many plane waves on one operator (the multi-RHS entry the reference's
+// GSTC driver has, solve_gstc_dense_batch gstc_sie.hpp:214-239, for T-PMCHWT):
+// one assembly, one factorisation, one RHS column per source; each case
+// carries its own residual and coefficient maxima, `metrics` the shared
+// timings of the batch.
+struct DielectricBatchResult {
+    std::vector<DielectricResult> cases;
+    DielectricMetrics metrics;
+};
+
+DielectricBatchResult solve_dielectric_dense_batch(const mesh::MeshData& mesh, const DielectricOptions& options,
+                                                   const std::vector<excitation::PreparedExcitation>& sources);
 
 }  // namespace solvers
 }  // namespace sie_mom

@@ -16,8 +16,6 @@
 namespace sie_mom {
 namespace solvers {
 
-enum class SolvePrecision { fp64, mixed };
-
 struct GstcSieOptions {
     real_t frequency_hz = real_t(0);
     IsotropicMedium interior;

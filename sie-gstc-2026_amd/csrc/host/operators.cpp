@@ -73,6 +73,16 @@ std::vector<double> MixedLuSolver::solve(Op op, int nrhs, const complex_t* rhs, 
     if (op == Op::T) throw std::invalid_argument("MixedLuSolver supports op N and C");
     std::vector<double> rel(nrhs);
     std::vector<int> its(nrhs);
+    if (op == Op::N && nrhs > 4) {
+        // many columns: blocked refinement sweeps (one FP64 GEMM residual and one
+        // multi-RHS FP32 solve per sweep)
+        detail::check(sie_ir_solve((int)op, n_, nrhs, reinterpret_cast<const double*>(A_), ld_,
+                                   static_cast<const float*>(lu32_), n_, pivots_, reinterpret_cast<const double*>(rhs),
+                                   ldb > 0 ? ldb : n_, reinterpret_cast<double*>(x), ldx > 0 ? ldx : n_, tol, maxit,
+                                   rel.data(), its.data(), stream_));
+        if (sweeps) *sweeps = its;
+        return rel;
+    }
     // GMRES-IR (FP32 factors applied in FP64 as the preconditioner), restart 20
     detail::check(sie_fgmres_ir_solve((int)op, n_, nrhs, reinterpret_cast<const double*>(A_), ld_,
                                       static_cast<const float*>(lu32_), n_, pivots_,

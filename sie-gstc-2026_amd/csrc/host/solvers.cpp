@@ -422,24 +422,27 @@ PecResult solve_pec_dense(const mesh::MeshData& mesh, const PecOptions& o, const
     return result;
 }
 
-DielectricResult solve_dielectric_dense(const mesh::MeshData& mesh, const DielectricOptions& o,
-                                        const excitation::PreparedExcitation& source) {
+DielectricBatchResult solve_dielectric_dense_batch(const mesh::MeshData& mesh, const DielectricOptions& o,
+                                                   const std::vector<excitation::PreparedExcitation>& sources) {
     validate_common(mesh, o.frequency_hz, o.interior, o.exterior, o.vacuum_permittivity, o.vacuum_permeability,
                     o.quadrature_rule, o.singularity_threshold_lambda0);
     if (o.formulation != DielectricFormulation::T_PMCHWT)
         throw std::invalid_argument("only the T-PMCHWT dielectric formulation is provided by this library");
-    source.validate_environment(o.frequency_hz, o.interior, o.exterior, o.vacuum_permittivity, o.vacuum_permeability);
+    if (sources.empty()) throw std::invalid_argument("at least one excitation is required");
+    for (const auto& src : sources)
+        src.validate_environment(o.frequency_hz, o.interior, o.exterior, o.vacuum_permittivity, o.vacuum_permeability);
     if (mesh.edges.size() > std::numeric_limits<std::size_t>::max() / 2) throw std::overflow_error("SIE matrix dimension overflows size_t");
     check_dimension(2 * mesh.edges.size());
     host::require_device();
-    DielectricResult result;
-    auto& mt = result.metrics;
+    DielectricBatchResult batch;
+    auto& mt = batch.metrics;
     mt.vertex_count = mesh.vertices.size();
     mt.face_count = mesh.faces.size();
     mt.edge_count = mesh.edges.size();
     mt.matrix_dimension = 2 * mesh.edges.size();
     mt.dense_matrix_bytes = mt.matrix_dimension * mt.matrix_dimension * sizeof(complex_t);
     const int n = (int)mt.matrix_dimension;
+    const int nrhs = (int)sources.size();
     const double omega = 2.0 * std::acos(-1.0) * o.frequency_hz;
     auto kof = [&](const IsotropicMedium& m) {
         return omega * std::sqrt(m.relative_permeability * o.vacuum_permeability * m.relative_permittivity *
@@ -450,41 +453,74 @@ DielectricResult solve_dielectric_dense(const mesh::MeshData& mesh, const Dielec
                                               o.exterior, o.vacuum_permeability, o.vacuum_permittivity, omega, thr);
     auto t0 = Clock::now();
     host::MeshHandle mh(mesh);
-    DeviceArray<complex_t> Z((size_t)n * n), x(n);
+    DeviceArray<complex_t> Z((size_t)n * n), X((size_t)n * nrhs);
     mt.device_preparation_seconds = seconds_since(t0);
     t0 = Clock::now();
     auto assemble = [&] {
         detail::check(sie_assemble_pmchwt(mh.get(), &p, reinterpret_cast<double*>(Z.data()), n, nullptr));
     };
     assemble();
-    const sie_plane_wave w = source.plane_wave();
-    detail::check(sie_rhs_plane_waves(mh.get(), SIE_RHS_PMCHWT, o.quadrature_rule, &w, 1, (float)o.vacuum_permeability,
-                                      (float)o.vacuum_permittivity, o.vacuum_permeability, o.vacuum_permittivity,
-                                      reinterpret_cast<double*>(x.data()), n, nullptr));
+    std::vector<sie_plane_wave> waves;
+    for (const auto& src : sources) waves.push_back(src.plane_wave());
+    detail::check(sie_rhs_plane_waves(mh.get(), SIE_RHS_PMCHWT, o.quadrature_rule, waves.data(), nrhs,
+                                      (float)o.vacuum_permeability, (float)o.vacuum_permittivity, o.vacuum_permeability,
+                                      o.vacuum_permittivity, reinterpret_cast<double*>(X.data()), n, nullptr));
     host::hip_ok(hipDeviceSynchronize(), "PMCHWT assembly");
     mt.assembly_seconds = seconds_since(t0);
-    std::vector<complex_t> b(n);
-    x.download(b.data(), n);
+    std::vector<complex_t> B((size_t)n * nrhs);
+    X.download(B.data(), B.size());
     t0 = Clock::now();
-    gpu::solve_dense_lu(n, Z.data(), x.data());
+    if (o.precision == SolvePrecision::fp64) {
+        gpu::DenseLuSolver lu(n);
+        lu.factorize(Z.data());
+        lu.solve(gpu::Op::N, nrhs, Z.data(), X.data(), n);
+    } else {
+        gpu::MixedLuSolver mixed;
+        mixed.factorize(Z.data(), n);
+        DeviceArray<complex_t> Bd((size_t)n * nrhs);
+        Bd.upload(B.data(), B.size());
+        const auto rel = mixed.solve(gpu::Op::N, nrhs, Bd.data(), n, X.data(), n, o.refinement_tolerance,
+                                     o.refinement_max_sweeps);
+        for (double r : rel)
+            if (!(r <= o.refinement_tolerance))
+                throw std::runtime_error("iterative refinement did not reach the requested residual (" +
+                                         std::to_string(r) + ")");
+    }
+    host::hip_ok(hipDeviceSynchronize(), "PMCHWT solve");
     mt.solve_seconds = seconds_since(t0);
     if (o.compute_residual) {
         t0 = Clock::now();
         assemble();
-        residual(n, Z, x.data(), b, host::host_norm(b), mt.absolute_residual, mt.relative_residual);
         mt.residual_computed = true;
-        mt.residual_seconds = seconds_since(t0);
     }
-    std::vector<complex_t> cur(n);
-    x.download(cur.data(), n);
     const double eta0 = std::sqrt(o.vacuum_permeability / o.vacuum_permittivity);
     const std::size_t ne = mesh.edges.size();
-    for (std::size_t e = 0; e < ne; ++e) cur[ne + e] *= eta0;
-    const auto mx = block_maxima(cur, ne, 2, "non-finite physical current coefficient at index ");
-    mt.maximum_j_coefficient = mx[0];
-    mt.maximum_m_coefficient = mx[1];
-    result.physical_currents = std::move(cur);
-    return result;
+    batch.cases.resize(nrhs);
+    for (int c = 0; c < nrhs; ++c) {
+        DielectricResult& r = batch.cases[c];
+        r.metrics = mt;
+        if (o.compute_residual) {
+            const std::vector<complex_t> b(B.begin() + (size_t)c * n, B.begin() + (size_t)(c + 1) * n);
+            residual(n, Z, X.data() + (size_t)c * n, b, host::host_norm(b), r.metrics.absolute_residual,
+                     r.metrics.relative_residual);
+        }
+        std::vector<complex_t> cur(n);
+        X.download(cur.data(), n, (size_t)c * n);
+        for (std::size_t e = 0; e < ne; ++e) cur[ne + e] *= eta0;
+        const auto mx = block_maxima(cur, ne, 2, "non-finite physical current coefficient at index ");
+        r.metrics.maximum_j_coefficient = mx[0];
+        r.metrics.maximum_m_coefficient = mx[1];
+        r.physical_currents = std::move(cur);
+    }
+    if (o.compute_residual) mt.residual_seconds = seconds_since(t0);
+    for (auto& r : batch.cases) r.metrics.residual_seconds = mt.residual_seconds;
+    return batch;
+}
+
+DielectricResult solve_dielectric_dense(const mesh::MeshData& mesh, const DielectricOptions& o,
+                                        const excitation::PreparedExcitation& source) {
+    auto batch = solve_dielectric_dense_batch(mesh, o, {source});
+    return std::move(batch.cases[0]);
 }
 
 }  // namespace solvers

@@ -1001,6 +1001,44 @@ __global__ void axpy_kernel(long long n, double s, const cx<T>* __restrict__ d, 
     x[i].im += s * (double)d[i].im;
 }
 
+// Per-column sums of squares of an n x nrhs block (one workgroup per column,
+// fixed-order tree reduction).
+__global__ void colsumsq_kernel(long long n, const c64* __restrict__ R, long long ldr, double* __restrict__ out) {
+    __shared__ double s[256];
+    const c64* col = R + (long long)blockIdx.x * ldr;
+    double a = 0;
+    for (long long i = threadIdx.x; i < n; i += 256) a += col[i].re * col[i].re + col[i].im * col[i].im;
+    s[threadIdx.x] = a;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = s[0];
+}
+
+// D32[:, c] = R[:, c] * sc[c]  (FP64 -> FP32, per-column scale)
+__global__ void scale_cols_f32_kernel(long long n, int nrhs, const c64* __restrict__ R, long long ldr,
+                                      const double* __restrict__ sc, c32* __restrict__ D, long long ldd) {
+    const long long tot = n * nrhs;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (long long)gridDim.x * blockDim.x) {
+        const long long c = t / n, r = t - c * n;
+        const c64 v = R[c * ldr + r];
+        D[c * ldd + r] = {(float)(v.re * sc[c]), (float)(v.im * sc[c])};
+    }
+}
+
+// X[:, c] += D[:, c] * g[c]  (FP32 correction into the FP64 iterate)
+__global__ void axpy_cols_kernel(long long n, int nrhs, const c32* __restrict__ D, long long ldd,
+                                 const double* __restrict__ g, c64* __restrict__ X, long long ldx) {
+    const long long tot = n * nrhs;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (long long)gridDim.x * blockDim.x) {
+        const long long c = t / n, r = t - c * n;
+        X[c * ldx + r].re += g[c] * (double)D[c * ldd + r].re;
+        X[c * ldx + r].im += g[c] * (double)D[c * ldd + r].im;
+    }
+}
+
 // r = b - r  (r holds Z x on entry)
 __global__ void residual_kernel(long long n, const c64* __restrict__ b, c64* __restrict__ r) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1808,6 +1846,68 @@ static void zgemv(int op, int m, int n, double alpha, const c64* A, long long ld
     SIE_HIP(hipGetLastError());
 }
 
+// Blocked FP64 refinement of Z X = B around FP32 LU factors (op N): per sweep
+// R = B - Z X (FP64 GEMM), column norms on the host, D = LU32^{-1} (R / |r_c|)
+// for all columns at once, X += |r_c| D. Stops when every column reaches tol.
+static double norm2(long long n, const c64* x, hipStream_t s);
+static void ir_blocked(int n, int nrhs, const c64* Z, long long ldz, const c32* LU, long long ldl, const int* ipiv,
+                       const c64* B, long long ldb, c64* X, long long ldx, double tol, int maxit, double* rel_out,
+                       int* it_out, hipStream_t s) {
+    DevBuf<c64> R((size_t)n * nrhs);
+    DevBuf<c32> D((size_t)n * nrhs);
+    DevBuf<double> nrm(nrhs), sc(nrhs), gain(nrhs);
+    std::vector<double> bn(nrhs), rn(nrhs), hs(nrhs), hg(nrhs);
+    const long long tot = (long long)n * nrhs;
+    const unsigned g = (unsigned)std::min<long long>((tot + 255) / 256, 1 << 16);
+    auto column_norms = [&](const c64* M, long long ld, std::vector<double>& out) {
+        colsumsq_kernel<<<nrhs, 256, 0, s>>>(n, M, ld, nrm.p);
+        SIE_HIP(hipGetLastError());
+        SIE_HIP(hipMemcpyAsync(out.data(), nrm.p, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
+        SIE_HIP(hipStreamSynchronize(s));
+        for (double& v : out) v = std::sqrt(v);
+    };
+    column_norms(B, ldb, bn);
+    for (int c = 0; c < nrhs; ++c)
+        if (!(bn[c] > 0) || !std::isfinite(bn[c]))
+            throw Error(SIE_ERR_NONFINITE, "right-hand side norm must be finite and nonzero");
+    for (int c = 0; c < nrhs; ++c) SIE_HIP(hipMemsetAsync(X + (long long)c * ldx, 0, sizeof(c64) * n, s));
+    for (int c = 0; c < nrhs; ++c)
+        SIE_HIP(hipMemcpyAsync(R.p + (size_t)c * n, B + (long long)c * ldb, sizeof(c64) * n, hipMemcpyDeviceToDevice, s));
+    rn = bn;
+    std::vector<int> its(nrhs, 0);
+    std::vector<char> done(nrhs, 0);
+    for (int it = 0; it < maxit; ++it) {
+        int active = 0;
+        for (int c = 0; c < nrhs; ++c) {
+            if (!done[c] && rn[c] / bn[c] <= tol) done[c] = 1;
+            // converged columns get a zero correction (their iterate is kept)
+            hs[c] = done[c] ? 0.0 : 1.0 / rn[c];
+            hg[c] = done[c] ? 0.0 : rn[c];
+            if (!done[c]) { ++active; ++its[c]; }
+        }
+        if (!active) break;
+        SIE_HIP(hipMemcpyAsync(sc.p, hs.data(), sizeof(double) * nrhs, hipMemcpyHostToDevice, s));
+        SIE_HIP(hipMemcpyAsync(gain.p, hg.data(), sizeof(double) * nrhs, hipMemcpyHostToDevice, s));
+        scale_cols_f32_kernel<<<g, 256, 0, s>>>(n, nrhs, R.p, n, sc.p, D.p, n);
+        SIE_HIP(hipGetLastError());
+        getrs<float>(0, n, nrhs, LU, ldl, ipiv, D.p, n, s);
+        axpy_cols_kernel<<<g, 256, 0, s>>>(n, nrhs, D.p, n, gain.p, X, ldx);
+        SIE_HIP(hipGetLastError());
+        // R = B - Z X
+        for (int c = 0; c < nrhs; ++c)
+            SIE_HIP(hipMemcpyAsync(R.p + (size_t)c * n, B + (long long)c * ldb, sizeof(c64) * n,
+                                   hipMemcpyDeviceToDevice, s));
+        gemm<double>(n, nrhs, n, -1.0, Z, ldz, X, ldx, 1.0, R.p, n, s, 0);
+        column_norms(R.p, n, rn);
+        for (int c = 0; c < nrhs; ++c)
+            if (!std::isfinite(rn[c])) throw Error(SIE_ERR_NONFINITE, "iterative refinement diverged (non-finite residual)");
+    }
+    for (int c = 0; c < nrhs; ++c) {
+        if (rel_out) rel_out[c] = rn[c] / bn[c];
+        if (it_out) it_out[c] = its[c];
+    }
+}
+
 static double norm2(long long n, const c64* x, hipStream_t s) {
     const int nb = 512;
     double* part = (double*)ws(0).get(nb * sizeof(double));
@@ -1927,6 +2027,14 @@ int sie_ir_solve(int op, int n, int nrhs, const double* Z, long long ldz, const 
     require(op == 0 || op == 2, "op must be 0 (N) or 2 (C)");
     require(n > 0 && nrhs > 0 && ldz >= n && ldl >= n && ldb >= n && ldx >= n, "invalid IR dimensions");
     hipStream_t s = (hipStream_t)stream;
+    if (op == 0 && nrhs > 4) {
+        // many right-hand sides: every sweep is one FP64 GEMM residual and one
+        // multi-RHS FP32 solve on the matrix cores, columns scaled to unit norm
+        ir_blocked(n, nrhs, (const c64*)Z, ldz, (const c32*)LU, ldl, ipiv, (const c64*)B, ldb, (c64*)X, ldx, tol,
+                   maxit, rel_residual, iterations, s);
+        set_last_error("");
+        return SIE_OK;
+    }
     DevBuf<c64> r(n);
     DevBuf<c32> d32(n);
     const unsigned g = (unsigned)((n + 255) / 256);

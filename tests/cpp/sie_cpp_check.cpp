@@ -4,6 +4,7 @@
 //                              source validation, OBJ loader, binary formats.
 //   solve <dir> <freq> <rule>  GPU solves on an icosphere (level 1); writes
 //                              currents for comparison with the oracle.
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -225,6 +226,47 @@ static int solve(const std::string& dir, double f, int rule) {
     auto die = solvers::solve_dielectric_dense(sphere, dopt, srcd);
     dump(dir + "/pmchwt.bin", die.physical_currents);
     CHECK(die.metrics.relative_residual < 1e-12 && die.metrics.maximum_m_coefficient > 0);
+    {
+        // PMCHWT batch: FP64 cases match the single solves; six waves in mixed
+        // precision (blocked refinement) agree with FP64
+        std::vector<excitation::PreparedExcitation> srcs;
+        for (int wv = 0; wv < 6; ++wv) {
+            excitation::PlaneWaveExcitation pw;
+            const double th = 0.4 * wv;
+            pw.propagation_direction = make_vec3(std::sin(th), 0, std::cos(th));
+            pw.electric_amplitude = (wv % 2) ? make_vec3(0, 1, 0) : make_vec3(std::cos(th), 0, -std::sin(th));
+            srcs.push_back(excitation::prepare_source(pw, f, dopt.interior, dopt.exterior, dopt.vacuum_permittivity,
+                                                      dopt.vacuum_permeability));
+        }
+        srcs[0] = srcd;
+        auto b64 = solvers::solve_dielectric_dense_batch(sphere, dopt, srcs);
+        CHECK(b64.cases.size() == 6);
+        {
+            // six columns take the blocked multi-RHS solve: equal to the single solve to rounding
+            double num = 0, den = 0;
+            for (std::size_t i = 0; i < die.physical_currents.size(); ++i) {
+                num += std::norm(b64.cases[0].physical_currents[i] - die.physical_currents[i]);
+                den += std::norm(die.physical_currents[i]);
+            }
+            CHECK(std::sqrt(num / den) < 1e-12);
+        }
+        solvers::DielectricOptions dm = dopt;
+        dm.precision = solvers::SolvePrecision::mixed;
+        auto bmx = solvers::solve_dielectric_dense_batch(sphere, dm, srcs);
+        double worst = 0;
+        for (int c = 0; c < 6; ++c) {
+            double num = 0, den = 0;
+            for (std::size_t i = 0; i < b64.cases[c].physical_currents.size(); ++i) {
+                num += std::norm(bmx.cases[c].physical_currents[i] - b64.cases[c].physical_currents[i]);
+                den += std::norm(b64.cases[c].physical_currents[i]);
+            }
+            worst = std::max(worst, std::sqrt(num / den));
+            CHECK(bmx.cases[c].metrics.relative_residual < 1e-11);
+        }
+        CHECK(worst < 1e-10);
+        CHECK(throws<std::invalid_argument>([&] { solvers::solve_dielectric_dense_batch(sphere, dopt, {}); },
+                                            "at least one excitation"));
+    }
     // operator level: GMRES on the stored PEC matrix against the LU solution
     {
         const int n = 120;

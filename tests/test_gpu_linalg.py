@@ -163,6 +163,28 @@ def test_fgmres_ir_restarts(sm):
         sm.linalg.ir_solve(Zd, LU32, lu.pivots, to_dev(b), method="cg")
 
 
+@pytest.mark.parametrize("variant", ["4m", "f16x3"])
+def test_ir_solve_many_rhs(sm, variant):
+    """More than four right-hand sides take the blocked refinement (one FP64
+    GEMM residual and one multi-RHS FP32 solve per sweep); columns of very
+    different scale converge independently."""
+    rng = np.random.default_rng(12)
+    n, nrhs = 900, 24
+    A = crandn(rng, n, n) + 5 * np.sqrt(n) * np.eye(n)
+    b = crandn(rng, n, nrhs) * np.logspace(-20, 20, nrhs)[None, :]
+    Zd = to_dev(A)
+    LU32 = sm.linalg.to_precision(Zd, 32)
+    lu = sm.linalg.DenseLuSolver(variant=variant)
+    lu.factorize(LU32)
+    X, rr, its = sm.linalg.ir_solve(Zd, LU32, lu.pivots, to_dev(b), tol=1e-13, maxit=30)
+    assert np.all(rr < 1e-13), rr
+    assert its.max() <= 12
+    xr = np.linalg.solve(A, b)
+    x = X.cpu().numpy()
+    for c in range(nrhs):
+        assert np.linalg.norm(x[:, c] - xr[:, c]) / np.linalg.norm(xr[:, c]) < 1e-12
+
+
 def test_convert_grid_stride(sm):
     # more elements than one capped dispatch covers: exercises the grid-stride loop
     n = 5000
