@@ -43,7 +43,7 @@ def well_conditioned(rng, n):
 
 
 @pytest.mark.parametrize("prec,variant", [(64, "4m"), (32, "4m"), (32, "f16x3")])
-@pytest.mark.parametrize("n", [16, 100, 300, 513, 1024, 2500])  # 2500: several 1024-column super-panels
+@pytest.mark.parametrize("n", [16, 100, 300, 513, 1024, 2500, 5000])  # 2500, 5000: several 1024- and 4096-column blocks
 def test_lu_solve(sm, prec, variant, n):
     rng = np.random.default_rng(n)
     A = crandn(rng, n, n)  # general (pivoting exercised)
