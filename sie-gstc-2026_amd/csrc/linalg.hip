@@ -899,6 +899,76 @@ __global__ __launch_bounds__(256) void trsm_small_kernel(int w, const cx<T>* __r
         if (i < w) b[i] = x[i];
 }
 
+// In-place solve of a w x w (w <= WM) triangular system on ncols right-hand columns:
+//   uplo 0: unit lower L (forward), 1: non-unit upper U (backward),
+//   2: conj-transposed non-unit upper U^H (forward), 3: conj-transposed unit lower L^H (backward).
+// The triangle T lives at A (column-major, lda); B at B (ldb). This form,
+// for few columns: WM lanes per column, lane i owning row i; at step k the
+// solved x_k is broadcast within the lane group and every lane below
+// (forward) or above (backward) subtracts its multiple, so a column costs WM
+// dependent steps of one complex FMA (latency) instead of WM^2 / 2 serial
+// FMAs in one lane. trsm_small_kernel (one lane per column) does less LDS and
+// shuffle work per column and wins when the columns fill the device.
+template <typename T, int WM, int UPLO>
+__global__ __launch_bounds__(256) void trsm_lanes_kernel(int w, const cx<T>* __restrict__ Tm, long long lda,
+                                                         cx<T>* __restrict__ B, long long ldb, int ncols) {
+    constexpr int TL = WM * WM / 256;  // triangle entries per thread for the LDS load
+    constexpr int CPB = 256 / WM;      // columns per workgroup
+    __shared__ cx<T> sT[WM * WM];      // column-major w x w, zero-padded
+    {
+        cx<T> v[TL];
+#pragma unroll
+        for (int q = 0; q < TL; ++q) {
+            const int t = q * 256 + threadIdx.x, col = t / WM, row = t % WM;
+            v[q] = (col < w && row < w) ? Tm[(long long)col * lda + row] : cx<T>{0, 0};
+        }
+#pragma unroll
+        for (int q = 0; q < TL; ++q) sT[q * 256 + threadIdx.x] = v[q];
+    }
+    __syncthreads();
+    const int i = threadIdx.x % WM;
+    const int c = blockIdx.x * CPB + threadIdx.x / WM;
+    // the multipliers this lane applies at step k: row i of the operator
+    //   0: L(i, k)   1: U(i, k)   2: conj(U(k, i))   3: conj(L(k, i))
+    cx<T> m[WM];
+#pragma unroll
+    for (int k = 0; k < WM; ++k) {
+        const cx<T> t = (UPLO < 2) ? sT[k * WM + i] : sT[i * WM + k];
+        m[k] = (UPLO < 2) ? t : cx<T>{t.re, -t.im};
+    }
+    cx<T>* b = B + (long long)c * ldb;
+    const bool live = c < ncols && i < w;
+    cx<T> x = live ? b[i] : cx<T>{0, 0};
+    auto divd = [](cx<T> v, cx<T> d) {
+        const T q = d.re * d.re + d.im * d.im;
+        return cx<T>{(v.re * d.re + v.im * d.im) / q, (v.im * d.re - v.re * d.im) / q};
+    };
+    if constexpr (UPLO == 0 || UPLO == 2) {  // forward
+#pragma unroll
+        for (int k = 0; k < WM; ++k) {
+            if (k >= w) break;
+            if (UPLO == 2 && i == k) x = divd(x, m[k]);  // diagonal conj(U(k, k))
+            const T xr = __shfl(x.re, k, WM), xi = __shfl(x.im, k, WM);
+            if (i > k) {
+                x.re -= m[k].re * xr - m[k].im * xi;
+                x.im -= m[k].re * xi + m[k].im * xr;
+            }
+        }
+    } else {  // backward
+#pragma unroll
+        for (int k = WM - 1; k >= 0; --k) {
+            if (k >= w) continue;
+            if (UPLO == 1 && i == k) x = divd(x, m[k]);  // diagonal U(k, k)
+            const T xr = __shfl(x.re, k, WM), xi = __shfl(x.im, k, WM);
+            if (i < k) {
+                x.re -= m[k].re * xr - m[k].im * xi;
+                x.im -= m[k].re * xi + m[k].im * xr;
+            }
+        }
+    }
+    if (live) b[i] = x;
+}
+
 // ================================================================= GEMV
 // Partial y_part[chunk][i] = sum_{j in chunk} A(i, j) x(j), op N.
 template <typename T, typename TX>
@@ -1395,12 +1465,22 @@ static void trsm_small(int w, const cx<T>* Tm, long long lda, cx<T>* B, long lon
     if (ncols <= 0 || w <= 0) return;
     constexpr int WM = 32;  // panel width (FP32) and the multi-RHS solve block
     require(w <= WM, "internal: triangular block wider than the register tile");
-    const unsigned g = (unsigned)((ncols + 255) / 256);
-    switch (uplo) {
-        case 0: trsm_small_kernel<T, WM, 0><<<g, 256, 0, s>>>(w, Tm, lda, B, ldb, ncols); break;
-        case 1: trsm_small_kernel<T, WM, 1><<<g, 256, 0, s>>>(w, Tm, lda, B, ldb, ncols); break;
-        case 2: trsm_small_kernel<T, WM, 2><<<g, 256, 0, s>>>(w, Tm, lda, B, ldb, ncols); break;
-        default: trsm_small_kernel<T, WM, 3><<<g, 256, 0, s>>>(w, Tm, lda, B, ldb, ncols); break;
+    if (ncols <= 4096) {  // latency-bound: WM lanes per column
+        const unsigned g = (unsigned)((ncols + 256 / WM - 1) / (256 / WM));
+        switch (uplo) {
+            case 0: trsm_lanes_kernel<T, WM, 0><<<g, 256, 0, s>>>(w, Tm, lda, B, ldb, ncols); break;
+            case 1: trsm_lanes_kernel<T, WM, 1><<<g, 256, 0, s>>>(w, Tm, lda, B, ldb, ncols); break;
+            case 2: trsm_lanes_kernel<T, WM, 2><<<g, 256, 0, s>>>(w, Tm, lda, B, ldb, ncols); break;
+            default: trsm_lanes_kernel<T, WM, 3><<<g, 256, 0, s>>>(w, Tm, lda, B, ldb, ncols); break;
+        }
+    } else {  // many columns: one lane per column
+        const unsigned g = (unsigned)((ncols + 255) / 256);
+        switch (uplo) {
+            case 0: trsm_small_kernel<T, WM, 0><<<g, 256, 0, s>>>(w, Tm, lda, B, ldb, ncols); break;
+            case 1: trsm_small_kernel<T, WM, 1><<<g, 256, 0, s>>>(w, Tm, lda, B, ldb, ncols); break;
+            case 2: trsm_small_kernel<T, WM, 2><<<g, 256, 0, s>>>(w, Tm, lda, B, ldb, ncols); break;
+            default: trsm_small_kernel<T, WM, 3><<<g, 256, 0, s>>>(w, Tm, lda, B, ldb, ncols); break;
+        }
     }
     SIE_HIP(hipGetLastError());
 }
