@@ -373,14 +373,18 @@ __global__ void split_f16_blk_kernel(int R, int K, int RB, const c32* __restrict
 // 8 XCDs, each with its own L2: give every XCD a contiguous range of tiles and
 // walk that range in groups of GM tile rows, column by column, so that the
 // blocks resident on one XCD share their A and B panels in its L2.
-template <bool SWZ>
+// Tile rows per group of the 256 x 128 kernel: its 32 co-resident workgroups
+// per XCD then cover 4 x 8 tiles, a 1024 x 1024 block of C whose A and B
+// panels are equally long (the 128 x 128 kernel's 64 cover 8 x 8 with GM = 8).
+constexpr int kWideGroupRows = 4;
+
+template <bool SWZ, int GM>
 __device__ __forceinline__ void f16x3_tile(int id, int nbm, int nbn, int& tm, int& tn) {
     if (!SWZ) {
         tm = id % nbm;
         tn = id / nbm;
         return;
     }
-    constexpr int GM = 8;
     const int total = nbm * nbn;
     const int per = total >> 3, rem = total & 7, x = id & 7;
     const int lin = x * per + min(x, rem) + (id >> 3);
@@ -497,7 +501,7 @@ __global__ __launch_bounds__(256, MINB) void cgemm_f16x3_kernel(int M, int N, in
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     int tm, tn;
-    f16x3_tile<SWZ>(blockIdx.x, (M + BM - 1) / BM, (N + BN - 1) / BN, tm, tn);
+    f16x3_tile<SWZ, 8>(blockIdx.x, (M + BM - 1) / BM, (N + BN - 1) / BN, tm, tn);
     const int i0 = tm * BM, j0 = tn * BN;
     const int KB = (K + 15) / 16;
     const size_t planeA = (size_t)KB * M * 16, planeB = (size_t)KB * N * 16;
@@ -577,7 +581,7 @@ __global__ __launch_bounds__(512, 1) void cgemm_f16x3_r3_kernel(int M, int N, in
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 3, wn = wave >> 2;
     int tm, tn;
-    f16x3_tile<SWZ>(blockIdx.x, (M + BM - 1) / BM, (N + BN - 1) / BN, tm, tn);
+    f16x3_tile<SWZ, kWideGroupRows>(blockIdx.x, (M + BM - 1) / BM, (N + BN - 1) / BN, tm, tn);
     const int i0 = tm * BM, j0 = tn * BN;
     const int KB = (K + 15) / 16;
     // Operands in the tile-blocked layout of split_f16_blk_kernel: the K slice
@@ -667,6 +671,10 @@ __device__ __forceinline__ bool better(double v, int i, double bv, int bi) {
 // P points at A(k1, k1) (column-major, lda); the panel is M rows x W columns.
 // ipiv[j] receives the 0-based panel row swapped with row j.
 // pub: [W][G] candidate rows (W values each) ; cand: [W][G] ; drow: [W][W].
+// Per column: local argmax, publication of the workgroup's candidate (value,
+// row index, row values) and of row j, one grid barrier; then every workgroup
+// loads the candidates together with row j, picks the global winner (largest
+// |re|+|im|, lowest index on ties), loads its row, swaps and updates its rows.
 template <typename T, int W, int BS>
 __global__ __launch_bounds__(BS) void panel_getf2_kernel(int M, int w, cx<T>* __restrict__ P, long long lda,
                                                          int* __restrict__ ipiv, Cand<T>* __restrict__ cand,
@@ -684,6 +692,7 @@ __global__ __launch_bounds__(BS) void panel_getf2_kernel(int M, int w, cx<T>* __
     __shared__ int si[BS / 64];
     __shared__ int swin;
     __shared__ cx<T> sprow[W];
+    __shared__ cx<T> sdrow[W];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 
     for (int j = 0; j < w; ++j) {
@@ -725,7 +734,9 @@ __global__ __launch_bounds__(BS) void panel_getf2_kernel(int M, int w, cx<T>* __
         }
         // 2. grid barrier
         grid_barrier(counter, (unsigned)(G * (j + 1)));
-        // 3. global winner (every workgroup computes the same)
+        // 3. global winner (every workgroup computes the same); row j is
+        // loaded in the same round trip as the candidates
+        if (threadIdx.x < W) sdrow[threadIdx.x] = drow[(size_t)j * W + threadIdx.x];
         if (threadIdx.x < 64) {
             double bv = -1.0;
             int bi = 0x7fffffff, bg = 0;
@@ -762,7 +773,7 @@ __global__ __launch_bounds__(BS) void panel_getf2_kernel(int M, int w, cx<T>* __
         // swap: position p takes old row j, position j takes the pivot row
         if (own && r == p && p != j) {
 #pragma unroll
-            for (int k = 0; k < W; ++k) a[k] = drow[(size_t)j * W + k];
+            for (int k = 0; k < W; ++k) a[k] = sdrow[k];
         }
         if (own && r == j) {
 #pragma unroll
@@ -787,6 +798,163 @@ __global__ __launch_bounds__(BS) void panel_getf2_kernel(int M, int w, cx<T>* __
                 }
             }
         }
+        // sprow and sdrow are rewritten after the next grid barrier, which
+        // every thread of the workgroup reaches after this column's reads
+    }
+    if (own) {
+#pragma unroll
+        for (int k = 0; k < W; ++k)
+            if (k < w) P[(long long)k * lda + r] = a[k];
+    }
+}
+
+// 64-bit pivot key of a candidate of the FP32 panel: |re| + |im| (a
+// non-negative float, whose bit pattern orders as an unsigned integer) above
+// the complement of the row index, so that the largest key is the largest
+// modulus with the lowest row index (izamax's choice, as better() above).
+// Zero means "not yet published": the keys are cleared before each launch and
+// every published key has a nonzero low word (row indices are below 2^31).
+__device__ __forceinline__ unsigned long long pivot_key(float v, int idx) {
+    return ((unsigned long long)__float_as_uint(v) << 32) | (0xFFFFFFFFu - (unsigned)idx);
+}
+
+// The FP32 panel with the candidates as keys: per column, each workgroup
+// publishes its candidate row, then its key with release semantics into the
+// column's own slot; every workgroup polls the G keys of the column until all
+// are published (this wait is the grid-wide synchronisation: no counter),
+// takes the largest and loads that row together with row j. Compared with the
+// counter barrier this takes the atomic and the separate candidate load off
+// each column's critical path. Slots are per column, so no slot is reused
+// within a launch. Requires all G workgroups co-resident (cooperative launch).
+template <int W, int BS>
+__global__ __launch_bounds__(BS) void panel_getf2_keyed_kernel(int M, int w, cx<float>* __restrict__ P,
+                                                               long long lda, int* __restrict__ ipiv,
+                                                               unsigned long long* __restrict__ keys,
+                                                               cx<float>* __restrict__ pub,
+                                                               cx<float>* __restrict__ drow,
+                                                               int* __restrict__ info, int info_base) {
+    const int G = gridDim.x;
+    const int r = blockIdx.x * BS + threadIdx.x;  // owned row
+    const bool own = r < M;
+    cx<float> a[W];
+#pragma unroll
+    for (int k = 0; k < W; ++k) a[k] = (own && k < w) ? P[(long long)k * lda + r] : cx<float>{0, 0};
+
+    __shared__ unsigned long long skey[BS / 64];
+    __shared__ int swin;
+    __shared__ cx<float> sprow[W];
+    __shared__ cx<float> sdrow[W];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+
+    for (int j = 0; j < w; ++j) {
+        // 1. local argmax over rows >= j of column j (static register indexing)
+        cx<float> aj = {0, 0};
+#pragma unroll
+        for (int k = 0; k < W; ++k)
+            if (k == j) aj = a[k];
+        unsigned long long key = pivot_key(0.f, 0x7fffffff);
+        if (own && r >= j) key = pivot_key(cabs1(aj), r);
+        for (int off = 32; off > 0; off >>= 1) {
+            const unsigned long long o = __shfl_xor(key, off);
+            key = o > key ? o : key;
+        }
+        if (lane == 0) skey[wv] = key;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long b = skey[0];
+            for (int q = 1; q < BS / 64; ++q) b = skey[q] > b ? skey[q] : b;
+            skey[0] = b;
+            swin = (int)(0xFFFFFFFFu - (unsigned)(b & 0xFFFFFFFFu));
+        }
+        __syncthreads();
+        // 2. publish the local winner's row and, by its owner, row j; then the key
+        if (own && r == swin) {
+#pragma unroll
+            for (int k = 0; k < W; ++k) pub[((size_t)j * G + blockIdx.x) * W + k] = a[k];
+        }
+        if (own && r == j) {
+#pragma unroll
+            for (int k = 0; k < W; ++k) drow[(size_t)j * W + k] = a[k];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+            __hip_atomic_store(&keys[(size_t)j * G + blockIdx.x], skey[0], __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        // 3. wait for every workgroup's key of column j; the largest wins
+        if (threadIdx.x < 64) {
+            unsigned long long best = 0;
+            int bg = 0;
+            for (;;) {
+                bool ready = true;
+                best = 0;
+                for (int g = lane; g < G; g += 64) {
+                    const unsigned long long kv =
+                        __hip_atomic_load(&keys[(size_t)j * G + g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ready = ready && kv != 0;
+                    if (kv > best) { best = kv; bg = g; }
+                }
+                if (__all(ready)) break;
+                __builtin_amdgcn_s_sleep(1);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            for (int off = 32; off > 0; off >>= 1) {
+                const unsigned long long ob = __shfl_xor(best, off);
+                const int og = __shfl_xor(bg, off);
+                if (ob > best) { best = ob; bg = og; }
+            }
+            if (threadIdx.x == 0) {
+                const int bi = (int)(0xFFFFFFFFu - (unsigned)(best & 0xFFFFFFFFu));
+                swin = (best >> 32) != 0 ? (bi | (bg << 20)) : -1;
+            }
+        }
+        __syncthreads();
+        const int code = swin;
+        if (code < 0) {
+            // exactly singular column: record, keep going without a swap (zgetf2 semantics)
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                ipiv[j] = j;
+                if (*info == 0) *info = info_base + j + 1;
+            }
+            continue;
+        }
+        const int p = code & 0xFFFFF, pg = code >> 20;
+        // the pivot row and row j, shared by the workgroup (one round trip)
+        if (threadIdx.x < W) {
+            sprow[threadIdx.x] = pub[((size_t)j * G + pg) * W + threadIdx.x];
+            sdrow[threadIdx.x] = drow[(size_t)j * W + threadIdx.x];
+        }
+        __syncthreads();
+        if (blockIdx.x == 0 && threadIdx.x == 0) ipiv[j] = p;
+        // swap: position p takes old row j, position j takes the pivot row
+        if (own && r == p && p != j) {
+#pragma unroll
+            for (int k = 0; k < W; ++k) a[k] = sdrow[k];
+        }
+        if (own && r == j) {
+#pragma unroll
+            for (int k = 0; k < W; ++k) a[k] = sprow[k];
+        }
+        // 4. scale and rank-1 update of rows below j
+        const cx<float> pj = sprow[j];
+        if (own && r > j) {
+            cx<float> ajj = {0, 0};
+#pragma unroll
+            for (int k = 0; k < W; ++k)
+                if (k == j) ajj = a[k];
+            const float d = pj.re * pj.re + pj.im * pj.im;
+            const cx<float> inv = {pj.re / d, -pj.im / d};
+            const cx<float> l = {ajj.re * inv.re - ajj.im * inv.im, ajj.re * inv.im + ajj.im * inv.re};
+#pragma unroll
+            for (int k = 0; k < W; ++k) {
+                if (k == j) a[k] = l;
+                if (k > j) {
+                    a[k].re -= l.re * sprow[k].re - l.im * sprow[k].im;
+                    a[k].im -= l.re * sprow[k].im + l.im * sprow[k].re;
+                }
+            }
+        }
+        // skey, swin, sprow and sdrow are rewritten in the next column only
+        // after __syncthreads that every thread reaches after these reads
     }
     if (own) {
 #pragma unroll
@@ -1500,13 +1668,16 @@ constexpr int kPanelBS = 512;
 template <typename T>
 static void panel(int M, int w, cx<T>* P, long long lda, int* ipiv, int* d_info, int info_base, hipStream_t s) {
     constexpr int W = PanelCfg<T>::W;
+    constexpr bool keyed = std::is_same<T, float>::value;
+    const void* kern = keyed ? (const void*)panel_getf2_keyed_kernel<W, kPanelBS>
+                             : (const void*)panel_getf2_kernel<T, W, kPanelBS>;
     const int G = (M + kPanelBS - 1) / kPanelBS;
     static int max_coresident = -1;
     if (max_coresident < 0) {
         int dev = 0, ncu = 0, per = 0;
         SIE_HIP(hipGetDevice(&dev));
         SIE_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-        SIE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, panel_getf2_kernel<T, W, kPanelBS>, kPanelBS, 0));
+        SIE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, kPanelBS, 0));
         max_coresident = ncu * per;
     }
     if (G > max_coresident)
@@ -1521,11 +1692,19 @@ static void panel(int M, int w, cx<T>* P, long long lda, int* ipiv, int* d_info,
     cx<T>* pub = (cx<T>*)(wsp + cand_b);
     cx<T>* drow = (cx<T>*)(wsp + cand_b + pub_b);
     unsigned int* counter = (unsigned int*)(wsp + cand_b + pub_b + drow_b);
-    SIE_HIP(hipMemsetAsync(counter, 0, sizeof(unsigned int), s));
-    void* args[] = {&M, &w, &P, &lda, &ipiv, &cand, &pub, &drow, &counter, &d_info, &info_base};
     prof().begin(PROF_PANEL, s);
-    SIE_HIP(hipLaunchCooperativeKernel((const void*)panel_getf2_kernel<T, W, kPanelBS>, dim3(G), dim3(kPanelBS),
-                                       args, 0, s));
+    if (keyed) {
+        // the W x G pivot keys (8 bytes each) reuse the candidate slots, cleared
+        static_assert(sizeof(Cand<float>) >= sizeof(unsigned long long), "key slot");
+        unsigned long long* keys = (unsigned long long*)cand;
+        SIE_HIP(hipMemsetAsync(keys, 0, (size_t)W * G * sizeof(unsigned long long), s));
+        void* args[] = {&M, &w, &P, &lda, &ipiv, &keys, &pub, &drow, &d_info, &info_base};
+        SIE_HIP(hipLaunchCooperativeKernel(kern, dim3(G), dim3(kPanelBS), args, 0, s));
+    } else {
+        SIE_HIP(hipMemsetAsync(counter, 0, sizeof(unsigned int), s));
+        void* args[] = {&M, &w, &P, &lda, &ipiv, &cand, &pub, &drow, &counter, &d_info, &info_base};
+        SIE_HIP(hipLaunchCooperativeKernel(kern, dim3(G), dim3(kPanelBS), args, 0, s));
+    }
     prof().end(PROF_PANEL, s, 8.0 * M * w * w / 2, 2.0 * sizeof(cx<T>) * M * w);
 }
 
