@@ -55,6 +55,18 @@ struct cplx { double re, im; };
 
 __device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
+// 1/sqrt(x) for finite x > 0: the hardware estimate and one third-order
+// correction, y (1 + e/2 + 3e^2/8) with e = 1 - x y^2 (the device library's
+// rsqrt without its zero/infinity handling).
+__device__ __forceinline__ double rsqrt_pos(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    const double e = fma(-x * y, y, 1.0);
+    return fma(y * e, fma(e, 0.375, 0.5), y);
+}
+
+// Largest quadrature rule of the far-pair kernel (its weights are staged in LDS).
+constexpr int kMaxFarQ = 128;
+
 // Float centroid distance < float threshold, without FMA contraction so the
 // predicate is bit-identical to the reference (float dist = length(c_m - c_n)).
 __device__ __forceinline__ bool is_near(const double* __restrict__ c, int nf, int m, int n, float thr) {
@@ -105,8 +117,15 @@ template <int NK, bool SYM>
 __global__ __launch_bounds__(256) void far_pairs_kernel(
     int nf, int n0, int nc, int Q, const double* __restrict__ c, const double* __restrict__ o,
     const double* __restrict__ len, const double* __restrict__ bq, const float* __restrict__ qxy,
-    const float* __restrict__ qw, double k1, double k2, float thr, cplx* __restrict__ W) {
+    const float* __restrict__ qw, const double* __restrict__ sctab, double k1, double k2, float thr,
+    cplx* __restrict__ W) {
     constexpr int NS = NK * 18;
+    // the e^{j 2 pi i / 256} table and the source weights w_q / (4 pi), shared by the workgroup
+    __shared__ double stab[2 * kSinCosTabN];
+    __shared__ double swq[kMaxFarQ];
+    for (int t = threadIdx.x; t < 2 * kSinCosTabN; t += blockDim.x) stab[t] = sctab[t];
+    for (int t = threadIdx.x; t < Q; t += blockDim.x) swq[t] = (double)qw[t] * kInv4Pi;
+    __syncthreads();
     const int m = blockIdx.x * 64 + (threadIdx.x & 63);
     const int nl = uniform(blockIdx.y * 4 + (threadIdx.x >> 6));
     if (nl >= nc) return;
@@ -150,16 +169,16 @@ __global__ __launch_bounds__(256) void far_pairs_kernel(
         }
         for (int q = 0; q < Q; ++q) {
             const double bx = bqn[3 * q], by = bqn[3 * q + 1], bz = bqn[3 * q + 2];
-            const double wq = qw[q];
             const double Rx = P[0] - bx, Ry = P[1] - by, Rz = P[2] - bz;
             const double R2 = Rx * Rx + Ry * Ry + Rz * Rz;
-            const double iR = rsqrt(R2);
+            // R2 > 0 on far pairs; lanes of other pairs compute values that are not stored
+            const double iR = rsqrt_pos(R2);
             const double Rm = R2 * iR;
-            const double gsc = wq * kInv4Pi * iR;
+            const double gsc = swq[q] * iR;
 #pragma unroll
             for (int w = 0; w < NK; ++w) {
                 double s, co;
-                sincos_moderate(kk[w] * Rm, &s, &co);
+                sincos_tab(kk[w] * Rm, stab, &s, &co);
                 // w_q G = w_q e^{-jkR} / (4 pi R)
                 const double Gr = co * gsc, Gi = -s * gsc;
                 // w_q g = w_q G (1/R + jk) / R   (grad'G = g R_vec)
@@ -998,6 +1017,7 @@ struct Assembler {
     QuadDev quad;
     DevBuf<double> bq;
     int bq_rule = 0;
+    DevBuf<double> sctab;  // e^{j 2 pi i / 256}: (cos, sin) pairs for sincos_tab
     DevBuf<cplx> W;
     DevBuf<int2> near;
     DevBuf<unsigned int> count;
@@ -1013,6 +1033,12 @@ struct Assembler {
 
     void prepare(int rule, hipStream_t s) {
         quad_upload(quad, rule, s);
+        if (!sctab.p) {
+            double h[2 * kSinCosTabN];
+            sincos_table_fill(h);
+            sctab.alloc(2 * kSinCosTabN);
+            SIE_HIP(hipMemcpy(sctab.p, h, sizeof(h), hipMemcpyHostToDevice));
+        }
         if (bq_rule != rule) {
             bq.alloc((size_t)mesh->nf * quad.n * 3);
             const int tot = mesh->nf * quad.n;
@@ -1041,14 +1067,15 @@ struct Assembler {
         SIE_HIP(hipMemcpyAsync(h_count, count.p, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
         // far pairs
         dim3 grid((nf + 63) / 64, (nc + 3) / 4);
+        require(Q <= kMaxFarQ, "quadrature rule has more points than the far-pair kernel stages");
         prof().begin(PROF_FAR_PAIRS, s);
         if (sym) {
             require(n0 == 0 && nc == nf, "symmetric far pairs need every source face in one chunk");
             far_pairs_kernel<NK, true><<<grid, 256, 0, s>>>(nf, n0, nc, Q, mesh->d_c.p, mesh->d_o.p, mesh->d_len.p,
-                                                            bq.p, quad.xy.p, quad.w.p, k1, k2, thr, W.p);
+                                                            bq.p, quad.xy.p, quad.w.p, sctab.p, k1, k2, thr, W.p);
         } else {
             far_pairs_kernel<NK, false><<<grid, 256, 0, s>>>(nf, n0, nc, Q, mesh->d_c.p, mesh->d_o.p, mesh->d_len.p,
-                                                             bq.p, quad.xy.p, quad.w.p, k1, k2, thr, W.p);
+                                                             bq.p, quad.xy.p, quad.w.p, sctab.p, k1, k2, thr, W.p);
         }
         // flops of the moment-factored point-pair update (kFarPairFlops per pair and wavenumber),
         // over the pairs integrated (m > n only when symmetric)

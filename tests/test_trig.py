@@ -1,5 +1,6 @@
-"""sincos_moderate (csrc/trig.hpp) against long-double sin/cos on the host:
-the phase factor of every Green's-function evaluation in the assembly."""
+"""sincos_moderate and sincos_tab (csrc/trig.hpp) against long-double sin/cos
+on the host: the phase factor of every Green's-function evaluation in the
+assembly (sincos_tab in the far-pair quadrature, sincos_moderate elsewhere)."""
 import os
 import shutil
 import subprocess
@@ -15,19 +16,27 @@ SRC = r'''
 #include <random>
 #include "trig.hpp"
 int main() {
+    double tab[2 * sie::kSinCosTabN];
+    sie::sincos_table_fill(tab);
     std::mt19937_64 g(7);
     std::uniform_real_distribution<double> U(-4000.0, 4000.0);
-    double es = 0, ec = 0;
+    double es = 0, ec = 0, ts = 0, tc = 0;
     for (int i = 0; i < 2000000; ++i) {
-        // random arguments, and arguments next to multiples of pi/2 (cancellation in the reduction)
-        const double x = i < 400000 ? (double)(i / 4 - 50000) * 1.5707963267948966 + (i % 4 - 1.5) * 1e-9 : U(g);
+        // random arguments, and arguments next to multiples of pi/2 and of pi/128
+        // (cancellation in the two reductions)
+        const double x = i < 400000   ? (double)(i / 4 - 50000) * 1.5707963267948966 + (i % 4 - 1.5) * 1e-9
+                         : i < 800000 ? (double)(i / 4 - 150000) * 0.02454369260617026 + (i % 4 - 1.5) * 1e-9
+                                      : U(g);
         double s, c;
         sie::sincos_moderate(x, &s, &c);
         const long double S = sinl((long double)x), C = cosl((long double)x);
         es = std::fmax(es, (double)fabsl((long double)s - S));
         ec = std::fmax(ec, (double)fabsl((long double)c - C));
+        sie::sincos_tab(x, tab, &s, &c);
+        ts = std::fmax(ts, (double)fabsl((long double)s - S));
+        tc = std::fmax(tc, (double)fabsl((long double)c - C));
     }
-    std::printf("%.6e %.6e\n", es, ec);
+    std::printf("%.6e %.6e %.6e %.6e\n", es, ec, ts, tc);
     return 0;
 }
 '''
@@ -40,6 +49,7 @@ def test_sincos_moderate_accuracy(tmp_path):
     exe = tmp_path / "t"
     subprocess.run(["g++", "-O2", "-std=c++17", f"-I{HDR}", str(src), "-o", str(exe)], check=True)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
-    es, ec = float(out[0]), float(out[1])
+    es, ec, ts, tc = (float(v) for v in out)
     # absolute error within one unit in the last place of 1.0
     assert es < 2.3e-16 and ec < 2.3e-16, (es, ec)
+    assert ts < 2.3e-16 and tc < 2.3e-16, (ts, tc)
