@@ -818,14 +818,32 @@ __device__ __forceinline__ unsigned long long pivot_key(float v, int idx) {
     return ((unsigned long long)__float_as_uint(v) << 32) | (0xFFFFFFFFu - (unsigned)idx);
 }
 
+// Largest grid of the keyed panel (one 512-thread workgroup per CU fits its
+// registers, so a co-resident grid has at most one workgroup per CU anyway).
+constexpr int kPanelKeyedMaxG = 256;
+
+// Agent-scope (device-coherent) 8-byte accesses of a complex float: these
+// bypass the XCD-private L2, so rows published this way are read back by
+// other XCDs without an L2 writeback or invalidate.
+__device__ __forceinline__ void store_agent(cx<float>* p, cx<float> v) {
+    __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), __builtin_bit_cast(unsigned long long, v),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ cx<float> load_agent(const cx<float>* p) {
+    return __builtin_bit_cast(cx<float>, __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p),
+                                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
 // The FP32 panel with the candidates as keys: per column, each workgroup
-// publishes its candidate row, then its key with release semantics into the
-// column's own slot; every workgroup polls the G keys of the column until all
-// are published (this wait is the grid-wide synchronisation: no counter),
-// takes the largest and loads that row together with row j. Compared with the
-// counter barrier this takes the atomic and the separate candidate load off
-// each column's critical path. Slots are per column, so no slot is reused
-// within a launch. Requires all G workgroups co-resident (cooperative launch).
+// publishes its candidate row, then its key into the column's own slot;
+// every workgroup polls the G keys of the column until all are published
+// (this wait is the grid-wide synchronisation: no counter), takes the largest
+// and loads that row together with row j. Rows, keys and their reads are
+// agent-scope accesses, and each writer waits for its row stores to complete
+// (s_waitcnt) before the workgroup barrier that precedes the key store, so a
+// published key implies a published row without an L2 writeback per column.
+// Slots are per column, so no slot is reused within a launch. Requires all G
+// workgroups co-resident (cooperative launch).
 template <int W, int BS>
 __global__ __launch_bounds__(BS) void panel_getf2_keyed_kernel(int M, int w, cx<float>* __restrict__ P,
                                                                long long lda, int* __restrict__ ipiv,
@@ -833,6 +851,8 @@ __global__ __launch_bounds__(BS) void panel_getf2_keyed_kernel(int M, int w, cx<
                                                                cx<float>* __restrict__ pub,
                                                                cx<float>* __restrict__ drow,
                                                                int* __restrict__ info, int info_base) {
+    static_assert(W % 8 == 0, "columns are handled in groups of 8");
+    constexpr int NG = W / 8;
     const int G = gridDim.x;
     const int r = blockIdx.x * BS + threadIdx.x;  // owned row
     const bool own = r < M;
@@ -847,11 +867,17 @@ __global__ __launch_bounds__(BS) void panel_getf2_keyed_kernel(int M, int w, cx<
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 
     for (int j = 0; j < w; ++j) {
-        // 1. local argmax over rows >= j of column j (static register indexing)
+        const int jg = j >> 3;  // group of 8 columns holding column j (uniform)
+        // 1. local argmax over rows >= j of column j (static register indexing
+        // within the group of column j)
         cx<float> aj = {0, 0};
 #pragma unroll
-        for (int k = 0; k < W; ++k)
-            if (k == j) aj = a[k];
+        for (int g = 0; g < NG; ++g)
+            if (g == jg) {
+#pragma unroll
+                for (int kk = 0; kk < 8; ++kk)
+                    if (8 * g + kk == j) aj = a[8 * g + kk];
+            }
         unsigned long long key = pivot_key(0.f, 0x7fffffff);
         if (own && r >= j) key = pivot_key(cabs1(aj), r);
         for (int off = 32; off > 0; off >>= 1) {
@@ -870,33 +896,44 @@ __global__ __launch_bounds__(BS) void panel_getf2_keyed_kernel(int M, int w, cx<
         // 2. publish the local winner's row and, by its owner, row j; then the key
         if (own && r == swin) {
 #pragma unroll
-            for (int k = 0; k < W; ++k) pub[((size_t)j * G + blockIdx.x) * W + k] = a[k];
+            for (int k = 0; k < W; ++k) store_agent(&pub[((size_t)j * G + blockIdx.x) * W + k], a[k]);
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         }
         if (own && r == j) {
 #pragma unroll
-            for (int k = 0; k < W; ++k) drow[(size_t)j * W + k] = a[k];
+            for (int k = 0; k < W; ++k) store_agent(&drow[(size_t)j * W + k], a[k]);
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         }
         __syncthreads();
         if (threadIdx.x == 0)
-            __hip_atomic_store(&keys[(size_t)j * G + blockIdx.x], skey[0], __ATOMIC_RELEASE,
+            __hip_atomic_store(&keys[(size_t)j * G + blockIdx.x], skey[0], __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
-        // 3. wait for every workgroup's key of column j; the largest wins
+        // 3. wait for every workgroup's key of column j; the largest wins.
+        // G <= kPanelKeyedMaxG: each lane polls its (up to) 4 keys with the
+        // loads of one round in flight together
         if (threadIdx.x < 64) {
             unsigned long long best = 0;
             int bg = 0;
             for (;;) {
+                unsigned long long kv[kPanelKeyedMaxG / 64];
+#pragma unroll
+                for (int u = 0; u < kPanelKeyedMaxG / 64; ++u) {
+                    const int g = lane + 64 * u;
+                    // 1: a ready key that loses to every published one
+                    kv[u] = g < G ? __hip_atomic_load(&keys[(size_t)j * G + g], __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT)
+                                  : 1ull;
+                }
                 bool ready = true;
                 best = 0;
-                for (int g = lane; g < G; g += 64) {
-                    const unsigned long long kv =
-                        __hip_atomic_load(&keys[(size_t)j * G + g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    ready = ready && kv != 0;
-                    if (kv > best) { best = kv; bg = g; }
+#pragma unroll
+                for (int u = 0; u < kPanelKeyedMaxG / 64; ++u) {
+                    ready = ready && kv[u] != 0;
+                    if (kv[u] > best) { best = kv[u]; bg = lane + 64 * u; }
                 }
                 if (__all(ready)) break;
                 __builtin_amdgcn_s_sleep(1);
             }
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             for (int off = 32; off > 0; off >>= 1) {
                 const unsigned long long ob = __shfl_xor(best, off);
                 const int og = __shfl_xor(bg, off);
@@ -918,10 +955,11 @@ __global__ __launch_bounds__(BS) void panel_getf2_keyed_kernel(int M, int w, cx<
             continue;
         }
         const int p = code & 0xFFFFF, pg = code >> 20;
-        // the pivot row and row j, shared by the workgroup (one round trip)
+        // the pivot row and row j, shared by the workgroup (one round trip,
+        // issued after every key of the column was seen)
         if (threadIdx.x < W) {
-            sprow[threadIdx.x] = pub[((size_t)j * G + pg) * W + threadIdx.x];
-            sdrow[threadIdx.x] = drow[(size_t)j * W + threadIdx.x];
+            sprow[threadIdx.x] = load_agent(&pub[((size_t)j * G + pg) * W + threadIdx.x]);
+            sdrow[threadIdx.x] = load_agent(&drow[(size_t)j * W + threadIdx.x]);
         }
         __syncthreads();
         if (blockIdx.x == 0 && threadIdx.x == 0) ipiv[j] = p;
@@ -934,22 +972,40 @@ __global__ __launch_bounds__(BS) void panel_getf2_keyed_kernel(int M, int w, cx<
 #pragma unroll
             for (int k = 0; k < W; ++k) a[k] = sprow[k];
         }
-        // 4. scale and rank-1 update of rows below j
+        // 4. scale and rank-1 update of rows below j, over the groups of
+        // columns >= j only
         const cx<float> pj = sprow[j];
         if (own && r > j) {
             cx<float> ajj = {0, 0};
 #pragma unroll
-            for (int k = 0; k < W; ++k)
-                if (k == j) ajj = a[k];
+            for (int g = 0; g < NG; ++g)
+                if (g == jg) {
+#pragma unroll
+                    for (int kk = 0; kk < 8; ++kk)
+                        if (8 * g + kk == j) ajj = a[8 * g + kk];
+                }
             const float d = pj.re * pj.re + pj.im * pj.im;
             const cx<float> inv = {pj.re / d, -pj.im / d};
             const cx<float> l = {ajj.re * inv.re - ajj.im * inv.im, ajj.re * inv.im + ajj.im * inv.re};
 #pragma unroll
-            for (int k = 0; k < W; ++k) {
-                if (k == j) a[k] = l;
-                if (k > j) {
-                    a[k].re -= l.re * sprow[k].re - l.im * sprow[k].im;
-                    a[k].im -= l.re * sprow[k].im + l.im * sprow[k].re;
+            for (int g = 0; g < NG; ++g) {
+                if (g > jg) {
+#pragma unroll
+                    for (int kk = 0; kk < 8; ++kk) {
+                        const int k = 8 * g + kk;
+                        a[k].re -= l.re * sprow[k].re - l.im * sprow[k].im;
+                        a[k].im -= l.re * sprow[k].im + l.im * sprow[k].re;
+                    }
+                } else if (g == jg) {
+#pragma unroll
+                    for (int kk = 0; kk < 8; ++kk) {
+                        const int k = 8 * g + kk;
+                        if (k == j) a[k] = l;
+                        if (k > j) {
+                            a[k].re -= l.re * sprow[k].re - l.im * sprow[k].im;
+                            a[k].im -= l.re * sprow[k].im + l.im * sprow[k].re;
+                        }
+                    }
                 }
             }
         }
@@ -1679,6 +1735,7 @@ static void panel(int M, int w, cx<T>* P, long long lda, int* ipiv, int* d_info,
         SIE_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
         SIE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, kPanelBS, 0));
         max_coresident = ncu * per;
+        if (keyed) max_coresident = std::min(max_coresident, kPanelKeyedMaxG);
     }
     if (G > max_coresident)
         throw Error(SIE_ERR_OVERFLOW, "panel height " + std::to_string(M) + " exceeds the co-resident capacity (" +
