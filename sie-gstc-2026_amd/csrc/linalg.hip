@@ -822,16 +822,34 @@ __device__ __forceinline__ unsigned long long pivot_key(float v, int idx) {
 // registers, so a co-resident grid has at most one workgroup per CU anyway).
 constexpr int kPanelKeyedMaxG = 256;
 
-// Agent-scope (device-coherent) 8-byte accesses of a complex float: these
-// bypass the XCD-private L2, so rows published this way are read back by
-// other XCDs without an L2 writeback or invalidate.
-__device__ __forceinline__ void store_agent(cx<float>* p, cx<float> v) {
-    __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), __builtin_bit_cast(unsigned long long, v),
-                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// Agent-scope (device-coherent) accesses of a complex value, 8 bytes at a
+// time: values published inside a launch (pivot rows of the panel, solution
+// blocks of the sweeps) are read by other workgroups, possibly on other XCDs,
+// without an L2 writeback or invalidate.
+template <typename T>
+__device__ __forceinline__ void store_agent_cx(cx<T>* p, cx<T> v) {
+    if constexpr (sizeof(T) == 8) {
+        __hip_atomic_store(reinterpret_cast<unsigned long long*>(&p->re), __builtin_bit_cast(unsigned long long, v.re),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(reinterpret_cast<unsigned long long*>(&p->im), __builtin_bit_cast(unsigned long long, v.im),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), __builtin_bit_cast(unsigned long long, v),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
-__device__ __forceinline__ cx<float> load_agent(const cx<float>* p) {
-    return __builtin_bit_cast(cx<float>, __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p),
+template <typename T>
+__device__ __forceinline__ cx<T> load_agent_cx(const cx<T>* p) {
+    if constexpr (sizeof(T) == 8) {
+        const unsigned long long re = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(&p->re),
+                                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long im = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(&p->im),
+                                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return cx<T>{__builtin_bit_cast(T, re), __builtin_bit_cast(T, im)};
+    } else {
+        return __builtin_bit_cast(cx<T>, __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p),
                                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    }
 }
 
 // The FP32 panel with the candidates as keys: per column, each workgroup
@@ -896,12 +914,12 @@ __global__ __launch_bounds__(BS) void panel_getf2_keyed_kernel(int M, int w, cx<
         // 2. publish the local winner's row and, by its owner, row j; then the key
         if (own && r == swin) {
 #pragma unroll
-            for (int k = 0; k < W; ++k) store_agent(&pub[((size_t)j * G + blockIdx.x) * W + k], a[k]);
+            for (int k = 0; k < W; ++k) store_agent_cx(&pub[((size_t)j * G + blockIdx.x) * W + k], a[k]);
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         }
         if (own && r == j) {
 #pragma unroll
-            for (int k = 0; k < W; ++k) store_agent(&drow[(size_t)j * W + k], a[k]);
+            for (int k = 0; k < W; ++k) store_agent_cx(&drow[(size_t)j * W + k], a[k]);
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         }
         __syncthreads();
@@ -958,8 +976,8 @@ __global__ __launch_bounds__(BS) void panel_getf2_keyed_kernel(int M, int w, cx<
         // the pivot row and row j, shared by the workgroup (one round trip,
         // issued after every key of the column was seen)
         if (threadIdx.x < W) {
-            sprow[threadIdx.x] = load_agent(&pub[((size_t)j * G + pg) * W + threadIdx.x]);
-            sdrow[threadIdx.x] = load_agent(&drow[(size_t)j * W + threadIdx.x]);
+            sprow[threadIdx.x] = load_agent_cx(&pub[((size_t)j * G + pg) * W + threadIdx.x]);
+            sdrow[threadIdx.x] = load_agent_cx(&drow[(size_t)j * W + threadIdx.x]);
         }
         __syncthreads();
         if (blockIdx.x == 0 && threadIdx.x == 0) ipiv[j] = p;
@@ -1414,7 +1432,7 @@ __global__ void diag_inv_kernel(int n, const cx<TF>* __restrict__ LU, long long 
 // thread in flight at once.
 constexpr int kSweepThreads = 1024;
 
-template <typename T, int B>
+template <typename T, int B, bool AGENT = false>
 __device__ __forceinline__ void trsv_block_solve(const cx<T>* __restrict__ M, const cx<T>* v, int nw, T (*spr)[B],
                                                  T (*spi)[B], cx<T>* __restrict__ out) {
     // out[i] = sum_j M[j B + i] v_j: M is the inverse (op N) or its conjugate
@@ -1439,7 +1457,8 @@ __device__ __forceinline__ void trsv_block_solve(const cx<T>* __restrict__ M, co
         T re = 0, im = 0;
 #pragma unroll
         for (int q = 0; q < P; ++q) { re += spr[q][t]; im += spi[q][t]; }
-        out[t] = cx<T>{re, im};
+        if constexpr (AGENT) store_agent_cx(out + t, cx<T>{re, im});
+        else out[t] = cx<T>{re, im};
     }
 }
 
@@ -1454,23 +1473,19 @@ __global__ void conj_transpose_blocks_kernel(long long total, const cx<T>* __res
     }
 }
 
+// sres (the workgroup's B rows of the right-hand side, rows [rb, rb + B), in
+// LDS) -= the factor tile of those rows and source columns [k0, k0 + w) times
+// sy: op 0 the tile of L or U itself (column-major reads), op 2 the tile of
+// U^H or L^H (row reads of the stored factor, conjugated). Each row's
+// products are summed in a fixed order, then subtracted once.
 template <typename TF, typename T, int B>
-__global__ __launch_bounds__(kSweepThreads) void trsv_update_kernel(int w, int k0, const cx<TF>* __restrict__ LU,
-                                                                    long long lda, const cx<T>* __restrict__ Y, int op,
-                                                                    cx<T>* __restrict__ vec, int r0, int r1, int nxt0,
-                                                                    int nw, const cx<T>* __restrict__ invn,
-                                                                    cx<T>* __restrict__ out) {
+__device__ __forceinline__ void sweep_tile(int op, int w, int k0, const cx<TF>* __restrict__ LU, long long lda, int rb,
+                                           int r1, const cx<T>* sy, cx<T>* sres, T (*spr)[B], T (*spi)[B]) {
     constexpr int P = kSweepThreads / B;   // op N: column groups
     constexpr int CG = B / P;              // columns per group
     constexpr int L = B / 4;               // op C: lanes per row, 4 entries each
     constexpr int RP = kSweepThreads / L;  // op C: rows per pass
-    __shared__ cx<T> sy[B];
-    __shared__ cx<T> sres[B];
-    __shared__ T spr[P][B], spi[P][B];
     const int t = threadIdx.x;
-    const int rb = r0 + blockIdx.x * B;
-    if (t < B) sy[t] = t < w ? Y[k0 + t] : cx<T>{0, 0};
-    __syncthreads();
     if (op == 0) {
         const int row = t % B, grp = t / B, r = rb + row;
         T ar = 0, ai = 0;
@@ -1494,16 +1509,10 @@ __global__ __launch_bounds__(kSweepThreads) void trsv_update_kernel(int w, int k
             T re = 0, im = 0;
 #pragma unroll
             for (int q = 0; q < P; ++q) { re += spr[q][t]; im += spi[q][t]; }
-            cx<T> v = vec[rb + t];
-            v.re -= re;
-            v.im -= im;
-            vec[rb + t] = v;
-            sres[t] = v;
+            sres[t].re -= re;
+            sres[t].im -= im;
         }
     } else {
-        // the workgroup's rows of vec, staged once
-        if (t < B) sres[t] = rb + t < r1 ? vec[rb + t] : cx<T>{0, 0};
-        __syncthreads();
         const int sub = t % L, rr = t / L;
 #pragma unroll
         for (int pass = 0; pass < B / RP; ++pass) {
@@ -1531,12 +1540,90 @@ __global__ __launch_bounds__(kSweepThreads) void trsv_update_kernel(int w, int k
                 sres[row].im -= ai;
             }
         }
-        __syncthreads();
-        if (t < B && rb + t < r1) vec[rb + t] = sres[t];
     }
-    if (rb != nxt0) return;
-    __syncthreads();
-    trsv_block_solve<T, B>(invn, sres, nw, spr, spi, out + nxt0);
+}
+
+// Bounded wait for a block of the solution published in the same launch by a
+// workgroup of lower index: that workgroup was dispatched earlier and waits
+// only on workgroups of even lower index, so the wait ends; the bound (about
+// seconds) only keeps a broken invariant from hanging the device.
+__device__ __forceinline__ void wait_block_flag(const unsigned int* f) {
+    for (unsigned it = 0; it < (1u << 22); ++it) {
+        if (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+        __builtin_amdgcn_s_sleep(2);
+    }
+}
+
+// Diagonal blocks solved per launch of the sweep kernel.
+constexpr int kSweepChain = 8;
+
+// One launch of a block triangular sweep: with the solution block kb known
+// (in out), solves the next D blocks (kb+1 .. kb+D forward, kb-1 .. kb-D
+// backward) and applies source blocks kb .. kb+-(D-1) to every row block
+// beyond them. Workgroup d < D owns block kb+-(1+d): it applies source blocks
+// kb .. kb+-d in order (waiting for each block solved in this launch), then
+// solves its block with the diagonal-block inverse and publishes it
+// (agent-scope stores, completed before its flag is set). The nbulk
+// workgroups >= D share the row blocks beyond (block i to workgroup D + i mod
+// nbulk) and apply the D source blocks one after the other, each as soon as
+// it is published; nbulk is at most the number of CUs minus D, so that every
+// workgroup is resident. Every row receives the same products in the same
+// order as with one source block per launch.
+template <typename TF, typename T, int B>
+__global__ __launch_bounds__(kSweepThreads) void trsv_chain_kernel(int n, int dir, int kb, int D, int nbulk, int op,
+                                                                   const cx<TF>* __restrict__ LU, long long lda,
+                                                                   cx<T>* __restrict__ vec, const cx<T>* __restrict__ inv,
+                                                                   cx<T>* __restrict__ out,
+                                                                   unsigned int* __restrict__ flags) {
+    constexpr int P = kSweepThreads / B;
+    __shared__ cx<T> sy[B];
+    __shared__ cx<T> sres[B];
+    __shared__ T spr[P][B], spi[P][B];
+    const int t = threadIdx.x;
+    const int wg = blockIdx.x;
+    const int nb = (n + B - 1) / B;
+    if (wg < D) {
+        const int rbk = dir > 0 ? kb + 1 + wg : kb - 1 - wg;
+        const int rb = rbk * B;
+        if (t < B) sres[t] = rb + t < n ? vec[rb + t] : cx<T>{0, 0};
+        for (int q = 0; q <= wg; ++q) {
+            const int src = dir > 0 ? kb + q : kb - q;
+            if (q > 0 && t == 0) wait_block_flag(&flags[src]);
+            __syncthreads();
+            const int w = min(B, n - src * B);
+            if (t < B) sy[t] = t < w ? load_agent_cx(&out[(size_t)src * B + t]) : cx<T>{0, 0};
+            __syncthreads();
+            sweep_tile<TF, T, B>(op, w, src * B, LU, lda, rb, n, sy, sres, spr, spi);
+        }
+        __syncthreads();
+        const int nw = min(B, n - rb);
+        trsv_block_solve<T, B, true>(inv + (size_t)rbk * B * B, sres, nw, spr, spi, out + rb);
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+        if (t == 0) __hip_atomic_store(&flags[rbk], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+    const int nbeyond = dir > 0 ? nb - (kb + 1 + D) : kb - D;
+    for (int q = 0; q < D; ++q) {
+        const int src = dir > 0 ? kb + q : kb - q;
+        if (q > 0 && t == 0) wait_block_flag(&flags[src]);
+        __syncthreads();
+        const int w = min(B, n - src * B);
+        if (t < B) sy[t] = t < w ? load_agent_cx(&out[(size_t)src * B + t]) : cx<T>{0, 0};
+        for (int i = wg - D; i < nbeyond; i += nbulk) {
+            const int rb = (dir > 0 ? kb + 1 + D + i : i) * B;
+            if (t < B) sres[t] = rb + t < n ? vec[rb + t] : cx<T>{0, 0};
+            __syncthreads();
+            sweep_tile<TF, T, B>(op, w, src * B, LU, lda, rb, n, sy, sres, spr, spi);
+            __syncthreads();
+            if (t < B && rb + t < n) vec[rb + t] = sres[t];
+        }
+    }
+}
+
+__global__ void zero_u32_kernel(int n, unsigned int* __restrict__ p) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
 }
 
 // out[k0 + i] = M vec[k0 + i] for one diagonal block (the first of a sweep).
@@ -1943,29 +2030,41 @@ static void note_factorisation(const void* A) {
 // invL / invU: diagonal-block inverses for op N, their conjugate transposes for op C.
 template <typename TF, typename T>
 static void record_sweeps(int op, int n, const cx<TF>* LU, long long lda, cx<T>* b, const cx<T>* invL,
-                          const cx<T>* invU, cx<T>* y, cx<T>* t, const int* perm, hipStream_t s) {
+                          const cx<T>* invU, cx<T>* y, cx<T>* t, const int* perm, unsigned int* flags,
+                          hipStream_t s) {
     constexpr int B = SweepCfg<TF>::B;
     const int nb = (n + B - 1) / B;
     const unsigned g = (unsigned)((n + 255) / 256);
     auto width = [&](int k) { return std::min(B, n - k); };
-    // forward sweep over blocks 0, B, ...: solve block k, update rows [k + B, n)
+    // bulk workgroups per launch: one per CU besides the kSweepChain chain
+    // workgroups (one 1024-thread workgroup fills a CU)
+    int dev = 0, ncu = 0;
+    SIE_HIP(hipGetDevice(&dev));
+    SIE_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const int bulk_max = std::max(1, ncu - kSweepChain);
+    // the per-block "solved" flags of both sweeps start at zero (a kernel, not
+    // a memset node, so that the sequence can be captured and replayed)
+    zero_u32_kernel<<<(2 * nb + 255) / 256, 256, 0, s>>>(2 * nb, flags);
+    // forward sweep over blocks 0, 1, ...: solve block 0, then kSweepChain blocks per launch
     auto forward = [&](const cx<T>* inv, int sop, cx<T>* vec, cx<T>* out) {
         trsv_diag_kernel<T, B><<<1, kSweepThreads, 0, s>>>(0, width(0), inv, sop, vec, out);
-        for (int k = 0; k + B < n; k += B) {
-            const int rows = n - (k + B);
-            trsv_update_kernel<TF, T, B><<<(unsigned)((rows + B - 1) / B), kSweepThreads, 0, s>>>(
-                width(k), k, LU, lda, out, sop, vec, k + B, n, k + B, width(k + B),
-                inv + (size_t)((k + B) / B) * B * B, out);
+        for (int kb = 0; kb < nb - 1; kb += kSweepChain) {
+            const int D = std::min(kSweepChain, nb - 1 - kb);
+            const int nbulk = std::min(std::max(0, nb - (kb + 1 + D)), bulk_max);
+            trsv_chain_kernel<TF, T, B><<<D + nbulk, kSweepThreads, 0, s>>>(n, 1, kb, D, nbulk, sop, LU, lda, vec,
+                                                                           inv, out, flags);
         }
     };
-    // backward sweep over blocks (nb-1)B, ..., 0: solve block k, update rows [0, k)
+    // backward sweep over blocks nb-1, ..., 0
     auto backward = [&](const cx<T>* inv, int sop, cx<T>* vec, cx<T>* out) {
         const int kl = (nb - 1) * B;
         trsv_diag_kernel<T, B><<<1, kSweepThreads, 0, s>>>(kl, width(kl), inv + (size_t)(nb - 1) * B * B, sop, vec,
                                                             out);
-        for (int k = kl; k > 0; k -= B) {
-            trsv_update_kernel<TF, T, B><<<(unsigned)(k / B), kSweepThreads, 0, s>>>(
-                width(k), k, LU, lda, out, sop, vec, 0, k, k - B, B, inv + (size_t)((k - B) / B) * B * B, out);
+        for (int kb = nb - 1; kb > 0; kb -= kSweepChain) {
+            const int D = std::min(kSweepChain, kb);
+            const int nbulk = std::min(std::max(0, kb - D), bulk_max);
+            trsv_chain_kernel<TF, T, B><<<D + nbulk, kSweepThreads, 0, s>>>(n, -1, kb, D, nbulk, sop, LU, lda, vec,
+                                                                           inv, out, flags + nb);
         }
     };
     if (op == 0) {
@@ -2016,7 +2115,8 @@ void getrs_vec(int op, int n, const cx<TF>* LU, long long lda, const int* ipiv, 
     constexpr int tkey = (int)(sizeof(TF) * 100 + sizeof(T));  // cache key of the type pair
     const int nb = (n + B - 1) / B;
     const size_t invsz = (size_t)nb * B * B * sizeof(cx<T>);
-    unsigned char* w = (unsigned char*)ws(5).get(4 * invsz + 2 * (size_t)n * sizeof(cx<T>) + (size_t)n * sizeof(int));
+    unsigned char* w = (unsigned char*)ws(5).get(4 * invsz + 2 * (size_t)n * sizeof(cx<T>) + (size_t)n * sizeof(int) +
+                                                 2 * (size_t)nb * sizeof(unsigned int) + 16);
     cx<T>* invL = (cx<T>*)w;
     cx<T>* invU = (cx<T>*)(w + invsz);
     cx<T>* invLh = (cx<T>*)(w + 2 * invsz);
@@ -2024,6 +2124,7 @@ void getrs_vec(int op, int n, const cx<TF>* LU, long long lda, const int* ipiv, 
     cx<T>* y = (cx<T>*)(w + 4 * invsz);
     cx<T>* t = y + n;
     int* perm = (int*)(t + n);
+    unsigned int* flags = (unsigned int*)(perm + n);  // 2 nb sweep flags
     // the row permutation of P^T, composed on the host from the swap sequence
     std::vector<int> hp(n), hpiv(n);
     SIE_HIP(hipMemcpyAsync(hpiv.data(), ipiv, sizeof(int) * n, hipMemcpyDeviceToHost, s));
@@ -2055,7 +2156,7 @@ void getrs_vec(int op, int n, const cx<TF>* LU, long long lda, const int* ipiv, 
     const cx<T>* iL = op == 0 ? invL : invLh;
     const cx<T>* iU = op == 0 ? invU : invUh;
     if (nb < 8) {  // short sequences: launch directly
-        record_sweeps<TF, T>(op, n, LU, lda, b, iL, iU, y, t, perm, s);
+        record_sweeps<TF, T>(op, n, LU, lda, b, iL, iU, y, t, perm, flags, s);
         return;
     }
     hipGraphExec_t exec = nullptr;
@@ -2076,7 +2177,7 @@ void getrs_vec(int op, int n, const cx<TF>* LU, long long lda, const int* ipiv, 
         if (!cache.capture) SIE_HIP(hipStreamCreateWithFlags(&cache.capture, hipStreamNonBlocking));
         hipGraph_t graph = nullptr;
         SIE_HIP(hipStreamBeginCapture(cache.capture, hipStreamCaptureModeThreadLocal));
-        record_sweeps<TF, T>(op, n, LU, lda, b, iL, iU, y, t, perm, cache.capture);
+        record_sweeps<TF, T>(op, n, LU, lda, b, iL, iU, y, t, perm, flags, cache.capture);
         SIE_HIP(hipStreamEndCapture(cache.capture, &graph));
         SIE_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
         SIE_HIP(hipGraphDestroy(graph));
