@@ -852,24 +852,55 @@ __device__ __forceinline__ cx<T> load_agent_cx(const cx<T>* p) {
     }
 }
 
+// A published row of the FP32 panel: each complex entry as two 64-bit words,
+// (re bits | tag << 32) and (im bits | tag << 32), with a tag unique to the
+// launch and column. A reader that sees every word carry its tag has the
+// whole row, so the writer need not wait for its stores to complete before
+// publishing its key: the reader re-reads the (rare) row not yet complete.
+__device__ __forceinline__ void publish_row_tagged(unsigned long long* __restrict__ dst, const cx<float>* a, int W,
+                                                   unsigned tag) {
+    const unsigned long long hi = (unsigned long long)tag << 32;
+    for (int k = 0; k < W; ++k) {
+        __hip_atomic_store(dst + 2 * k, hi | __float_as_uint(a[k].re), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(dst + 2 * k + 1, hi | __float_as_uint(a[k].im), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// One wave reads a tagged row of W <= 32 entries (2W words, one per lane)
+// into LDS, re-reading until every word carries the tag (bounded: the words
+// were stored before the key that led here was, so they arrive).
+__device__ __forceinline__ void read_row_tagged(const unsigned long long* __restrict__ src, int W, unsigned tag,
+                                                float* __restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    unsigned long long v = 0;
+    for (unsigned it = 0; it < (1u << 20); ++it) {
+        v = lane < 2 * W ? __hip_atomic_load(src + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                         : (unsigned long long)tag << 32;
+        if (__all((unsigned)(v >> 32) == tag)) break;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    if (lane < 2 * W) out[lane] = __uint_as_float((unsigned)v);
+}
+
 // The FP32 panel with the candidates as keys: per column, each workgroup
-// publishes its candidate row, then its key into the column's own slot;
-// every workgroup polls the G keys of the column until all are published
-// (this wait is the grid-wide synchronisation: no counter), takes the largest
-// and loads that row together with row j. Rows, keys and their reads are
-// agent-scope accesses, and each writer waits for its row stores to complete
-// (s_waitcnt) before the workgroup barrier that precedes the key store, so a
-// published key implies a published row without an L2 writeback per column.
-// Slots are per column, so no slot is reused within a launch. Requires all G
-// workgroups co-resident (cooperative launch).
+// publishes its candidate row (tagged words), then its key into the column's
+// own slot; every workgroup polls the G keys of the column until all are
+// published (this wait is the grid-wide synchronisation: no counter), takes
+// the largest and reads that row and row j, whose tags prove them complete.
+// Rows, keys and their reads are agent-scope (device-coherent) accesses, so
+// no L2 writeback or invalidate is needed per column. Slots are per column
+// and tags per launch, so no stale word is taken for a current one. Requires
+// all G workgroups co-resident (cooperative launch).
 template <int W, int BS>
 __global__ __launch_bounds__(BS) void panel_getf2_keyed_kernel(int M, int w, cx<float>* __restrict__ P,
                                                                long long lda, int* __restrict__ ipiv,
                                                                unsigned long long* __restrict__ keys,
-                                                               cx<float>* __restrict__ pub,
-                                                               cx<float>* __restrict__ drow,
-                                                               int* __restrict__ info, int info_base) {
-    static_assert(W % 8 == 0, "columns are handled in groups of 8");
+                                                               unsigned long long* __restrict__ pub,
+                                                               unsigned long long* __restrict__ drow,
+                                                               unsigned tag_base, int* __restrict__ info,
+                                                               int info_base) {
+    static_assert(W % 8 == 0 && W <= 32, "columns are handled in groups of 8, a row read by one wave");
     constexpr int NG = W / 8;
     const int G = gridDim.x;
     const int r = blockIdx.x * BS + threadIdx.x;  // owned row
@@ -885,6 +916,7 @@ __global__ __launch_bounds__(BS) void panel_getf2_keyed_kernel(int M, int w, cx<
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 
     for (int j = 0; j < w; ++j) {
+        const unsigned tag = tag_base | (unsigned)(j + 1);
         const int jg = j >> 3;  // group of 8 columns holding column j (uniform)
         // 1. local argmax over rows >= j of column j (static register indexing
         // within the group of column j)
@@ -912,17 +944,8 @@ __global__ __launch_bounds__(BS) void panel_getf2_keyed_kernel(int M, int w, cx<
         }
         __syncthreads();
         // 2. publish the local winner's row and, by its owner, row j; then the key
-        if (own && r == swin) {
-#pragma unroll
-            for (int k = 0; k < W; ++k) store_agent_cx(&pub[((size_t)j * G + blockIdx.x) * W + k], a[k]);
-            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        }
-        if (own && r == j) {
-#pragma unroll
-            for (int k = 0; k < W; ++k) store_agent_cx(&drow[(size_t)j * W + k], a[k]);
-            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        }
-        __syncthreads();
+        if (own && r == swin) publish_row_tagged(pub + ((size_t)j * G + blockIdx.x) * 2 * W, a, W, tag);
+        if (own && r == j) publish_row_tagged(drow + (size_t)j * 2 * W, a, W, tag);
         if (threadIdx.x == 0)
             __hip_atomic_store(&keys[(size_t)j * G + blockIdx.x], skey[0], __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
@@ -973,12 +996,9 @@ __global__ __launch_bounds__(BS) void panel_getf2_keyed_kernel(int M, int w, cx<
             continue;
         }
         const int p = code & 0xFFFFF, pg = code >> 20;
-        // the pivot row and row j, shared by the workgroup (one round trip,
-        // issued after every key of the column was seen)
-        if (threadIdx.x < W) {
-            sprow[threadIdx.x] = load_agent_cx(&pub[((size_t)j * G + pg) * W + threadIdx.x]);
-            sdrow[threadIdx.x] = load_agent_cx(&drow[(size_t)j * W + threadIdx.x]);
-        }
+        // the pivot row (wave 0) and row j (wave 1), shared by the workgroup
+        if (wv == 0) read_row_tagged(pub + ((size_t)j * G + pg) * 2 * W, W, tag, (float*)sprow);
+        else if (wv == 1) read_row_tagged(drow + (size_t)j * 2 * W, W, tag, (float*)sdrow);
         __syncthreads();
         if (blockIdx.x == 0 && threadIdx.x == 0) ipiv[j] = p;
         // swap: position p takes old row j, position j takes the pivot row
@@ -1829,8 +1849,10 @@ static void panel(int M, int w, cx<T>* P, long long lda, int* ipiv, int* d_info,
                                           std::to_string(max_coresident * kPanelBS) + " rows)");
     require(w >= 1 && w <= W, "internal: panel width out of range");
     const size_t cand_b = (size_t)W * G * sizeof(Cand<T>);
-    const size_t pub_b = (size_t)W * G * W * sizeof(cx<T>);
-    const size_t drow_b = (size_t)W * W * sizeof(cx<T>);
+    // rows: W cx<T> each, or 2W tagged 64-bit words in the FP32 keyed form
+    const size_t row_b = std::max(sizeof(cx<T>), 2 * sizeof(unsigned long long)) * W;
+    const size_t pub_b = (size_t)W * G * row_b;
+    const size_t drow_b = (size_t)W * row_b;
     unsigned char* wsp = (unsigned char*)ws(1).get(cand_b + pub_b + drow_b + 256);
     Cand<T>* cand = (Cand<T>*)wsp;
     cx<T>* pub = (cx<T>*)(wsp + cand_b);
@@ -1842,7 +1864,15 @@ static void panel(int M, int w, cx<T>* P, long long lda, int* ipiv, int* d_info,
         static_assert(sizeof(Cand<float>) >= sizeof(unsigned long long), "key slot");
         unsigned long long* keys = (unsigned long long*)cand;
         SIE_HIP(hipMemsetAsync(keys, 0, (size_t)W * G * sizeof(unsigned long long), s));
-        void* args[] = {&M, &w, &P, &lda, &ipiv, &keys, &pub, &drow, &d_info, &info_base};
+        // row tags: a launch number above the 6-bit column number (launch
+        // numbers wrap after 2^26 launches; never 0, the cleared state)
+        static unsigned launch_no = 0;
+        launch_no = (launch_no + 1) & ((1u << 26) - 1);
+        if (launch_no == 0) launch_no = 1;
+        unsigned tag_base = launch_no << 6;
+        unsigned long long* pubw = (unsigned long long*)pub;
+        unsigned long long* droww = (unsigned long long*)drow;
+        void* args[] = {&M, &w, &P, &lda, &ipiv, &keys, &pubw, &droww, &tag_base, &d_info, &info_base};
         SIE_HIP(hipLaunchCooperativeKernel(kern, dim3(G), dim3(kPanelBS), args, 0, s));
     } else {
         SIE_HIP(hipMemsetAsync(counter, 0, sizeof(unsigned int), s));
