@@ -45,52 +45,65 @@ typedef double doublex4 __attribute__((ext_vector_type(4)));
 namespace {
 
 // =================================================================== GEMM
-// FP32: 4 waves as 2x2, wave tile 64x64 complex = 2x2 MFMA 32x32 tiles.
-constexpr int G32_BM = 128, G32_BN = 128, G32_KB = 16;
+// FP32: 4 waves as 2x2, wave tile (32 TI) x (32 TI) complex = TI x TI MFMA
+// 32x32 tiles; block tile (64 TI)^2. TI = 2 (128 x 128) for long K; TI = 1
+// (64 x 64) for the LU's K = 32 updates, whose 128 x 128 tiles are too few
+// and too long (16k cycles of MFMA per wave) to fill the chip.
+constexpr int G32_KB = 16;
 
-template <int MODE>  // 0 = 4M, 1 = 3M
+template <int MODE, int TI>  // MODE 0 = 4M, 1 = 3M
 __global__ __launch_bounds__(256, 2) void cgemm_f32_kernel(int M, int N, int K, float alpha, float beta,
                                                            const c32* __restrict__ A, long long lda,
                                                            const c32* __restrict__ B, long long ldb,
                                                            c32* __restrict__ C, long long ldc) {
-    __shared__ c32 sA[2][G32_KB][G32_BM];
-    __shared__ c32 sB[2][G32_BN][G32_KB + 1];
+    constexpr int BM = 64 * TI, BN = 64 * TI;
+    constexpr int LA = G32_KB * BM / 256, LB = BN * G32_KB / 256;  // elements per thread and K tile
+    __shared__ c32 sA[2][G32_KB][BM];
+    __shared__ c32 sB[2][BN][G32_KB + 1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
-    const long long i0 = (long long)blockIdx.x * G32_BM, j0 = (long long)blockIdx.y * G32_BN;
+    const long long i0 = (long long)blockIdx.x * BM, j0 = (long long)blockIdx.y * BN;
 
     constexpr int NACC = MODE == 0 ? 2 : 3;
-    floatx16 acc[NACC][2][2];
+    floatx16 acc[NACC][TI][TI];
 #pragma unroll
     for (int a = 0; a < NACC; ++a)
 #pragma unroll
-        for (int ti = 0; ti < 2; ++ti)
+        for (int ti = 0; ti < TI; ++ti)
 #pragma unroll
-            for (int tj = 0; tj < 2; ++tj)
+            for (int tj = 0; tj < TI; ++tj)
 #pragma unroll
                 for (int v = 0; v < 16; ++v) acc[a][ti][tj][v] = 0.f;
 
-    c32 ra[8], rb[8];
+    c32 ra[LA], rb[LB];
     auto gload = [&](int k0) {
 #pragma unroll
-        for (int r = 0; r < 8; ++r) {
-            const int L = r * 256 + tid;
+        for (int r = 0; r < LA; ++r) {
             // A tile: [k][i], i fastest
-            const int k = L >> 7, i = L & 127;
+            const int L = r * 256 + tid;
+            const int k = L / BM, i = L % BM;
             const long long gi = i0 + i, gk = k0 + k;
             ra[r] = (gi < M && gk < K) ? A[gk * lda + gi] : c32{0.f, 0.f};
+        }
+#pragma unroll
+        for (int r = 0; r < LB; ++r) {
             // B tile: [j][k], k fastest
-            const int j = L >> 4, kk = L & 15;
+            const int L = r * 256 + tid;
+            const int j = L / G32_KB, kk = L % G32_KB;
             const long long gj = j0 + j, gkk = k0 + kk;
             rb[r] = (gj < N && gkk < K) ? B[gj * ldb + gkk] : c32{0.f, 0.f};
         }
     };
     auto lstore = [&](int buf) {
 #pragma unroll
-        for (int r = 0; r < 8; ++r) {
+        for (int r = 0; r < LA; ++r) {
             const int L = r * 256 + tid;
-            sA[buf][L >> 7][L & 127] = ra[r];
-            sB[buf][L >> 4][L & 15] = rb[r];
+            sA[buf][L / BM][L % BM] = ra[r];
+        }
+#pragma unroll
+        for (int r = 0; r < LB; ++r) {
+            const int L = r * 256 + tid;
+            sB[buf][L / G32_KB][L % G32_KB] = rb[r];
         }
     };
 
@@ -103,15 +116,17 @@ __global__ __launch_bounds__(256, 2) void cgemm_f32_kernel(int M, int N, int K, 
         if (kt + 1 < nk) gload((kt + 1) * G32_KB);
 #pragma unroll
         for (int kk = 0; kk < G32_KB; kk += 2) {
-            c32 af[2], bf[2];
+            c32 af[TI], bf[TI];
 #pragma unroll
-            for (int ti = 0; ti < 2; ++ti) af[ti] = sA[buf][kk + (lane >> 5)][wm * 64 + ti * 32 + (lane & 31)];
+            for (int ti = 0; ti < TI; ++ti)
+                af[ti] = sA[buf][kk + (lane >> 5)][wm * 32 * TI + ti * 32 + (lane & 31)];
 #pragma unroll
-            for (int tj = 0; tj < 2; ++tj) bf[tj] = sB[buf][wn * 64 + tj * 32 + (lane & 31)][kk + (lane >> 5)];
+            for (int tj = 0; tj < TI; ++tj)
+                bf[tj] = sB[buf][wn * 32 * TI + tj * 32 + (lane & 31)][kk + (lane >> 5)];
 #pragma unroll
-            for (int ti = 0; ti < 2; ++ti)
+            for (int ti = 0; ti < TI; ++ti)
 #pragma unroll
-                for (int tj = 0; tj < 2; ++tj) {
+                for (int tj = 0; tj < TI; ++tj) {
                     if (MODE == 0) {
                         acc[0][ti][tj] = __builtin_amdgcn_mfma_f32_32x32x2f32(bf[tj].re, af[ti].re, acc[0][ti][tj], 0, 0, 0);
                         acc[0][ti][tj] = __builtin_amdgcn_mfma_f32_32x32x2f32(-bf[tj].im, af[ti].im, acc[0][ti][tj], 0, 0, 0);
@@ -132,13 +147,13 @@ __global__ __launch_bounds__(256, 2) void cgemm_f32_kernel(int M, int N, int K, 
     }
     // epilogue: lane owns row i = ibase + lane%32, column j = jbase + 8(v/4) + 4(lane/32) + v%4
 #pragma unroll
-    for (int ti = 0; ti < 2; ++ti)
+    for (int ti = 0; ti < TI; ++ti)
 #pragma unroll
-        for (int tj = 0; tj < 2; ++tj)
+        for (int tj = 0; tj < TI; ++tj)
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
-                const long long i = i0 + wm * 64 + ti * 32 + (lane & 31);
-                const long long j = j0 + wn * 64 + tj * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                const long long i = i0 + wm * 32 * TI + ti * 32 + (lane & 31);
+                const long long j = j0 + wn * 32 * TI + tj * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
                 if (i < M && j < N) {
                     float re, im;
                     if (MODE == 0) {
@@ -1688,12 +1703,20 @@ template <>
 void gemm<float>(int M, int N, int K, double alpha, const c32* A, long long lda, const c32* B, long long ldb,
                  double beta, c32* C, long long ldc, hipStream_t s, int mode3m) {
     if (M <= 0 || N <= 0) return;
-    dim3 grid((M + G32_BM - 1) / G32_BM, (N + G32_BN - 1) / G32_BN);
     prof().begin(PROF_GEMM_F32, s);
-    if (mode3m)
-        cgemm_f32_kernel<1><<<grid, 256, 0, s>>>(M, N, K, (float)alpha, (float)beta, A, lda, B, ldb, C, ldc);
-    else
-        cgemm_f32_kernel<0><<<grid, 256, 0, s>>>(M, N, K, (float)alpha, (float)beta, A, lda, B, ldb, C, ldc);
+    if (K <= 64) {
+        dim3 grid((M + 63) / 64, (N + 63) / 64);
+        if (mode3m)
+            cgemm_f32_kernel<1, 1><<<grid, 256, 0, s>>>(M, N, K, (float)alpha, (float)beta, A, lda, B, ldb, C, ldc);
+        else
+            cgemm_f32_kernel<0, 1><<<grid, 256, 0, s>>>(M, N, K, (float)alpha, (float)beta, A, lda, B, ldb, C, ldc);
+    } else {
+        dim3 grid((M + 127) / 128, (N + 127) / 128);
+        if (mode3m)
+            cgemm_f32_kernel<1, 2><<<grid, 256, 0, s>>>(M, N, K, (float)alpha, (float)beta, A, lda, B, ldb, C, ldc);
+        else
+            cgemm_f32_kernel<0, 2><<<grid, 256, 0, s>>>(M, N, K, (float)alpha, (float)beta, A, lda, B, ldb, C, ldc);
+    }
     // complex-equivalent flops 8MNK; bytes: A and B once, C read + written
     prof().end(PROF_GEMM_F32, s, 8.0 * M * N * K, 8.0 * ((double)M * K + (double)K * N + 2.0 * M * N));
     SIE_HIP(hipGetLastError());
