@@ -692,10 +692,11 @@ __device__ __forceinline__ bool better(double v, int i, double bv, int bi) {
 // |re|+|im|, lowest index on ties), loads its row, swaps and updates its rows.
 template <typename T, int W, int BS>
 __global__ __launch_bounds__(BS) void panel_getf2_kernel(int M, int w, cx<T>* __restrict__ P, long long lda,
-                                                         int* __restrict__ ipiv, Cand<T>* __restrict__ cand,
-                                                         cx<T>* __restrict__ pub, cx<T>* __restrict__ drow,
-                                                         unsigned int* __restrict__ counter,
-                                                         int* __restrict__ info, int info_base) {
+                                                         int* __restrict__ ipiv, int ipiv_off,
+                                                         Cand<T>* __restrict__ cand, cx<T>* __restrict__ pub,
+                                                         cx<T>* __restrict__ drow, unsigned int* __restrict__ counter,
+                                                         unsigned int counter_base, int* __restrict__ info,
+                                                         int info_base) {
     const int G = gridDim.x;
     const int r = blockIdx.x * BS + threadIdx.x;  // owned row
     const bool own = r < M;
@@ -748,7 +749,7 @@ __global__ __launch_bounds__(BS) void panel_getf2_kernel(int M, int w, cx<T>* __
             for (int k = 0; k < W; ++k) drow[(size_t)j * W + k] = a[k];
         }
         // 2. grid barrier
-        grid_barrier(counter, (unsigned)(G * (j + 1)));
+        grid_barrier(counter, counter_base + (unsigned)(G * (j + 1)));
         // 3. global winner (every workgroup computes the same); row j is
         // loaded in the same round trip as the candidates
         if (threadIdx.x < W) sdrow[threadIdx.x] = drow[(size_t)j * W + threadIdx.x];
@@ -774,7 +775,7 @@ __global__ __launch_bounds__(BS) void panel_getf2_kernel(int M, int w, cx<T>* __
         if (code < 0) {
             // exactly singular column: record, keep going without a swap (zgetf2 semantics)
             if (blockIdx.x == 0 && threadIdx.x == 0) {
-                ipiv[j] = j;
+                ipiv[j] = j + ipiv_off;
                 if (*info == 0) *info = info_base + j + 1;
             }
             continue;
@@ -784,7 +785,7 @@ __global__ __launch_bounds__(BS) void panel_getf2_kernel(int M, int w, cx<T>* __
         if (threadIdx.x < W) sprow[threadIdx.x] = pub[((size_t)j * G + pg) * W + threadIdx.x];
         __syncthreads();
         const cx<T>* prow = sprow;
-        if (blockIdx.x == 0 && threadIdx.x == 0) ipiv[j] = p;
+        if (blockIdx.x == 0 && threadIdx.x == 0) ipiv[j] = p + ipiv_off;
         // swap: position p takes old row j, position j takes the pivot row
         if (own && r == p && p != j) {
 #pragma unroll
@@ -909,8 +910,9 @@ __device__ __forceinline__ void read_row_tagged(const unsigned long long* __rest
 // all G workgroups co-resident (cooperative launch).
 template <int W, int BS>
 __global__ __launch_bounds__(BS) void panel_getf2_keyed_kernel(int M, int w, cx<float>* __restrict__ P,
-                                                               long long lda, int* __restrict__ ipiv,
+                                                               long long lda, int* __restrict__ ipiv, int ipiv_off,
                                                                unsigned long long* __restrict__ keys,
+                                                               unsigned long long* __restrict__ keys_next, int nkeys,
                                                                unsigned long long* __restrict__ pub,
                                                                unsigned long long* __restrict__ drow,
                                                                unsigned tag_base, int* __restrict__ info,
@@ -918,6 +920,9 @@ __global__ __launch_bounds__(BS) void panel_getf2_keyed_kernel(int M, int w, cx<
     static_assert(W % 8 == 0 && W <= 32, "columns are handled in groups of 8, a row read by one wave");
     constexpr int NG = W / 8;
     const int G = gridDim.x;
+    // the key slots of the next launch (those of the previous one, which has
+    // completed) start cleared
+    for (int t = blockIdx.x * BS + threadIdx.x; t < nkeys; t += G * BS) keys_next[t] = 0ull;
     const int r = blockIdx.x * BS + threadIdx.x;  // owned row
     const bool own = r < M;
     cx<float> a[W];
@@ -1005,7 +1010,7 @@ __global__ __launch_bounds__(BS) void panel_getf2_keyed_kernel(int M, int w, cx<
         if (code < 0) {
             // exactly singular column: record, keep going without a swap (zgetf2 semantics)
             if (blockIdx.x == 0 && threadIdx.x == 0) {
-                ipiv[j] = j;
+                ipiv[j] = j + ipiv_off;
                 if (*info == 0) *info = info_base + j + 1;
             }
             continue;
@@ -1015,7 +1020,7 @@ __global__ __launch_bounds__(BS) void panel_getf2_keyed_kernel(int M, int w, cx<
         if (wv == 0) read_row_tagged(pub + ((size_t)j * G + pg) * 2 * W, W, tag, (float*)sprow);
         else if (wv == 1) read_row_tagged(drow + (size_t)j * 2 * W, W, tag, (float*)sdrow);
         __syncthreads();
-        if (blockIdx.x == 0 && threadIdx.x == 0) ipiv[j] = p;
+        if (blockIdx.x == 0 && threadIdx.x == 0) ipiv[j] = p + ipiv_off;
         // swap: position p takes old row j, position j takes the pivot row
         if (own && r == p && p != j) {
 #pragma unroll
@@ -1099,33 +1104,47 @@ __global__ void laswp_kernel(cx<T>* __restrict__ A, long long lda, int c0, int n
 // In-place solve of a w x w (w <= WM) triangular system on ncols right-hand columns:
 //   uplo 0: unit lower L (forward), 1: non-unit upper U (backward),
 //   2: conj-transposed non-unit upper U^H (forward), 3: conj-transposed unit lower L^H (backward).
-// The triangle T lives at A (column-major, lda); B at B (ldb). One lane per
-// column; the column is held in registers (fully unrolled, static indices).
+// The triangle T lives at A (column-major, lda); B at B (ldb). One wave per
+// 64 columns, one lane per column: the columns' leading WM rows are loaded
+// and stored coalesced (32 lanes per 256-byte column segment) through LDS, and
+// each lane solves its column in registers (fully unrolled, static indices).
 template <typename T, int WM, int UPLO>
-__global__ __launch_bounds__(256) void trsm_small_kernel(int w, const cx<T>* __restrict__ Tm, long long lda,
-                                                         cx<T>* __restrict__ B, long long ldb, int ncols) {
+__global__ __launch_bounds__(64) void trsm_small_kernel(int w, const cx<T>* __restrict__ Tm, long long lda,
+                                                        cx<T>* __restrict__ B, long long ldb, int ncols) {
+    static_assert(WM == 32, "two columns per coalesced load of a wave");
     constexpr int uplo = UPLO;
-    constexpr int TL = WM * WM / 256;  // triangle entries per thread
-    __shared__ cx<T> sT[WM * WM];  // column-major w x w
+    constexpr int TL = WM * WM / 64;  // triangle entries per lane
+    __shared__ cx<T> sT[WM * WM];     // column-major w x w
+    __shared__ cx<T> sB[64][WM + 1];  // the wave's columns, padded
+    const int lane = threadIdx.x;
+    const long long c0 = (long long)blockIdx.x * 64;
     {
-        // the whole triangle in one round trip: every load issued before any LDS store
+        // the triangle in one round trip: every load issued before any LDS store
         cx<T> v[TL];
 #pragma unroll
         for (int q = 0; q < TL; ++q) {
-            const int t = q * 256 + threadIdx.x, col = t / WM, row = t % WM;
+            const int t = q * 64 + lane, col = t / WM, row = t % WM;
             v[q] = (col < w && row < w) ? Tm[(long long)col * lda + row] : cx<T>{0, 0};
         }
 #pragma unroll
-        for (int q = 0; q < TL; ++q) sT[q * 256 + threadIdx.x] = v[q];
+        for (int q = 0; q < TL; ++q) sT[q * 64 + lane] = v[q];
+    }
+    {
+        // the 64 columns: load q covers columns 2q and 2q + 1
+        cx<T> u[WM];
+#pragma unroll
+        for (int q = 0; q < WM; ++q) {
+            const int col = 2 * q + (lane >> 5), row = lane & 31;
+            u[q] = (c0 + col < ncols && row < w) ? B[(c0 + col) * ldb + row] : cx<T>{0, 0};
+        }
+#pragma unroll
+        for (int q = 0; q < WM; ++q) sB[2 * q + (lane >> 5)][lane & 31] = u[q];
     }
     __syncthreads();
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncols) return;
-    cx<T>* b = B + (long long)c * ldb;
     auto at = [&](int i, int j) { return sT[j * WM + i]; };
     cx<T> x[WM];
 #pragma unroll
-    for (int i = 0; i < WM; ++i) x[i] = i < w ? b[i] : cx<T>{0, 0};
+    for (int i = 0; i < WM; ++i) x[i] = sB[lane][i];
     auto msub = [](cx<T>& s, cx<T> a, cx<T> v) {  // s -= a v
         s.re -= a.re * v.re - a.im * v.im;
         s.im -= a.re * v.im + a.im * v.re;
@@ -1172,8 +1191,13 @@ __global__ __launch_bounds__(256) void trsm_small_kernel(int w, const cx<T>* __r
                 if (i < w && k < w) msubc(x[i], at(k, i), x[k]);
     }
 #pragma unroll
-    for (int i = 0; i < WM; ++i)
-        if (i < w) b[i] = x[i];
+    for (int i = 0; i < WM; ++i) sB[lane][i] = x[i];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < WM; ++q) {
+        const int col = 2 * q + (lane >> 5), row = lane & 31;
+        if (c0 + col < ncols && row < w) B[(c0 + col) * ldb + row] = sB[2 * q + (lane >> 5)][lane & 31];
+    }
 }
 
 // In-place solve of a w x w (w <= WM) triangular system on ncols right-hand columns:
@@ -1827,13 +1851,13 @@ static void trsm_small(int w, const cx<T>* Tm, long long lda, cx<T>* B, long lon
             case 2: trsm_lanes_kernel<T, WM, 2><<<g, 256, 0, s>>>(w, Tm, lda, B, ldb, ncols); break;
             default: trsm_lanes_kernel<T, WM, 3><<<g, 256, 0, s>>>(w, Tm, lda, B, ldb, ncols); break;
         }
-    } else {  // many columns: one lane per column
-        const unsigned g = (unsigned)((ncols + 255) / 256);
+    } else {  // many columns: one lane per column, one wave per 64 columns
+        const unsigned g = (unsigned)((ncols + 63) / 64);
         switch (uplo) {
-            case 0: trsm_small_kernel<T, WM, 0><<<g, 256, 0, s>>>(w, Tm, lda, B, ldb, ncols); break;
-            case 1: trsm_small_kernel<T, WM, 1><<<g, 256, 0, s>>>(w, Tm, lda, B, ldb, ncols); break;
-            case 2: trsm_small_kernel<T, WM, 2><<<g, 256, 0, s>>>(w, Tm, lda, B, ldb, ncols); break;
-            default: trsm_small_kernel<T, WM, 3><<<g, 256, 0, s>>>(w, Tm, lda, B, ldb, ncols); break;
+            case 0: trsm_small_kernel<T, WM, 0><<<g, 64, 0, s>>>(w, Tm, lda, B, ldb, ncols); break;
+            case 1: trsm_small_kernel<T, WM, 1><<<g, 64, 0, s>>>(w, Tm, lda, B, ldb, ncols); break;
+            case 2: trsm_small_kernel<T, WM, 2><<<g, 64, 0, s>>>(w, Tm, lda, B, ldb, ncols); break;
+            default: trsm_small_kernel<T, WM, 3><<<g, 64, 0, s>>>(w, Tm, lda, B, ldb, ncols); break;
         }
     }
     SIE_HIP(hipGetLastError());
@@ -1851,13 +1875,53 @@ struct PanelCfg<double> {
 };
 constexpr int kPanelBS = 512;
 
+// Panel workspace of one factorisation (sized for its first, tallest panel):
+// two arrays of W x Gmax pivot keys (FP32: each launch uses one and clears the
+// other for the next launch), the candidates, published rows, row j and the
+// barrier counter (FP64: monotone over the factorisation, so never reset
+// between launches). Both are cleared once per factorisation.
+struct PanelState {
+    int gmax = 0;
+    int parity = 0;
+    unsigned int counter_base = 0;
+};
+static PanelState& panel_state() {
+    static PanelState st;
+    return st;
+}
 template <typename T>
-static void panel(int M, int w, cx<T>* P, long long lda, int* ipiv, int* d_info, int info_base, hipStream_t s) {
+static unsigned char* panel_workspace(int gmax, size_t* keys_b, size_t* cand_b, size_t* pub_b, size_t* drow_b) {
+    constexpr int W = PanelCfg<T>::W;
+    // rows: W cx<T> each, or 2W tagged 64-bit words in the FP32 keyed form
+    const size_t row_b = std::max(sizeof(cx<T>), 2 * sizeof(unsigned long long)) * W;
+    *keys_b = 2 * (size_t)W * gmax * sizeof(unsigned long long);
+    *cand_b = (size_t)W * gmax * sizeof(Cand<T>);
+    *pub_b = (size_t)W * gmax * row_b;
+    *drow_b = (size_t)W * row_b;
+    return (unsigned char*)ws(1).get(*keys_b + *cand_b + *pub_b + *drow_b + 256);
+}
+template <typename T>
+static void panel_begin(int n, hipStream_t s) {
+    PanelState& st = panel_state();
+    st.gmax = (n + kPanelBS - 1) / kPanelBS;
+    st.parity = 0;
+    st.counter_base = 0;
+    size_t keys_b, cand_b, pub_b, drow_b;
+    unsigned char* wsp = panel_workspace<T>(st.gmax, &keys_b, &cand_b, &pub_b, &drow_b);
+    SIE_HIP(hipMemsetAsync(wsp, 0, keys_b, s));
+    SIE_HIP(hipMemsetAsync(wsp + keys_b + cand_b + pub_b + drow_b, 0, sizeof(unsigned int), s));
+}
+
+// Factors the M x w panel at P; ipiv[j] receives the absolute row (panel row
+// + ipiv_off) swapped with row j.
+template <typename T>
+static void panel(int M, int w, cx<T>* P, long long lda, int* ipiv, int ipiv_off, int* d_info, int info_base,
+                  hipStream_t s) {
     constexpr int W = PanelCfg<T>::W;
     constexpr bool keyed = std::is_same<T, float>::value;
     const void* kern = keyed ? (const void*)panel_getf2_keyed_kernel<W, kPanelBS>
                              : (const void*)panel_getf2_kernel<T, W, kPanelBS>;
-    const int G = (M + kPanelBS - 1) / kPanelBS;
+    int G = (M + kPanelBS - 1) / kPanelBS;
     static int max_coresident = -1;
     if (max_coresident < 0) {
         int dev = 0, ncu = 0, per = 0;
@@ -1871,22 +1935,21 @@ static void panel(int M, int w, cx<T>* P, long long lda, int* ipiv, int* d_info,
         throw Error(SIE_ERR_OVERFLOW, "panel height " + std::to_string(M) + " exceeds the co-resident capacity (" +
                                           std::to_string(max_coresident * kPanelBS) + " rows)");
     require(w >= 1 && w <= W, "internal: panel width out of range");
-    const size_t cand_b = (size_t)W * G * sizeof(Cand<T>);
-    // rows: W cx<T> each, or 2W tagged 64-bit words in the FP32 keyed form
-    const size_t row_b = std::max(sizeof(cx<T>), 2 * sizeof(unsigned long long)) * W;
-    const size_t pub_b = (size_t)W * G * row_b;
-    const size_t drow_b = (size_t)W * row_b;
-    unsigned char* wsp = (unsigned char*)ws(1).get(cand_b + pub_b + drow_b + 256);
-    Cand<T>* cand = (Cand<T>*)wsp;
-    cx<T>* pub = (cx<T>*)(wsp + cand_b);
-    cx<T>* drow = (cx<T>*)(wsp + cand_b + pub_b);
-    unsigned int* counter = (unsigned int*)(wsp + cand_b + pub_b + drow_b);
+    PanelState& st = panel_state();
+    require(G <= st.gmax, "internal: panel taller than its factorisation");
+    size_t keys_b, cand_b, pub_b, drow_b;
+    unsigned char* wsp = panel_workspace<T>(st.gmax, &keys_b, &cand_b, &pub_b, &drow_b);
+    unsigned long long* keys = (unsigned long long*)wsp;
+    Cand<T>* cand = (Cand<T>*)(wsp + keys_b);
+    cx<T>* pub = (cx<T>*)(wsp + keys_b + cand_b);
+    cx<T>* drow = (cx<T>*)(wsp + keys_b + cand_b + pub_b);
+    unsigned int* counter = (unsigned int*)(wsp + keys_b + cand_b + pub_b + drow_b);
     prof().begin(PROF_PANEL, s);
     if (keyed) {
-        // the W x G pivot keys (8 bytes each) reuse the candidate slots, cleared
-        static_assert(sizeof(Cand<float>) >= sizeof(unsigned long long), "key slot");
-        unsigned long long* keys = (unsigned long long*)cand;
-        SIE_HIP(hipMemsetAsync(keys, 0, (size_t)W * G * sizeof(unsigned long long), s));
+        int nkeys = W * st.gmax;
+        unsigned long long* kcur = keys + (size_t)st.parity * nkeys;
+        unsigned long long* knext = keys + (size_t)(st.parity ^ 1) * nkeys;
+        st.parity ^= 1;
         // row tags: a launch number above the 6-bit column number (launch
         // numbers wrap after 2^26 launches; never 0, the cleared state)
         static unsigned launch_no = 0;
@@ -1895,20 +1958,17 @@ static void panel(int M, int w, cx<T>* P, long long lda, int* ipiv, int* d_info,
         unsigned tag_base = launch_no << 6;
         unsigned long long* pubw = (unsigned long long*)pub;
         unsigned long long* droww = (unsigned long long*)drow;
-        void* args[] = {&M, &w, &P, &lda, &ipiv, &keys, &pubw, &droww, &tag_base, &d_info, &info_base};
+        void* args[] = {&M, &w, &P, &lda, &ipiv, &ipiv_off, &kcur, &knext, &nkeys, &pubw, &droww, &tag_base,
+                        &d_info, &info_base};
         SIE_HIP(hipLaunchCooperativeKernel(kern, dim3(G), dim3(kPanelBS), args, 0, s));
     } else {
-        SIE_HIP(hipMemsetAsync(counter, 0, sizeof(unsigned int), s));
-        void* args[] = {&M, &w, &P, &lda, &ipiv, &cand, &pub, &drow, &counter, &d_info, &info_base};
+        unsigned int base = st.counter_base;
+        st.counter_base += (unsigned)(G * w);  // one arrival per workgroup and column
+        void* args[] = {&M, &w, &P, &lda, &ipiv, &ipiv_off, &cand, &pub, &drow, &counter, &base, &d_info,
+                        &info_base};
         SIE_HIP(hipLaunchCooperativeKernel(kern, dim3(G), dim3(kPanelBS), args, 0, s));
     }
     prof().end(PROF_PANEL, s, 8.0 * M * w * w / 2, 2.0 * sizeof(cx<T>) * M * w);
-}
-
-// ipiv: relative panel indices -> absolute (add k1) for the swaps applied later.
-__global__ void offset_ipiv(int* ipiv, int n, int off) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) ipiv[i] += off;
 }
 
 constexpr int kNB = 256;  // block of the right-looking steps of the innermost level
@@ -1987,8 +2047,7 @@ static void getrf_block(int n, cx<T>* A, long long lda, int c0, int c1, int* ipi
         for (int k1 = k0; k1 < k0 + nb; k1 += W) {
             const int w = std::min(W, k0 + nb - k1);
             cx<T>* P = A + (long long)k1 * lda + k1;
-            panel<T>(n - k1, w, P, lda, ipiv + k1, d_info, k1, s);
-            offset_ipiv<<<1, W, 0, s>>>(ipiv + k1, w, k1);
+            panel<T>(n - k1, w, P, lda, ipiv + k1, k1, d_info, k1, s);
             laswp<T>(A, lda, k0, k1 - k0, ipiv, k1, k1 + w, 0, false, s);
             laswp<T>(A, lda, k1 + w, k0 + nb - (k1 + w), ipiv, k1, k1 + w, 0, false, s);
             const int rest = k0 + nb - (k1 + w);
@@ -2046,6 +2105,7 @@ template <typename T>
 void getrf(int n, cx<T>* A, long long lda, int* ipiv, int* d_info, hipStream_t s, int variant) {
     note_factorisation(A);
     SIE_HIP(hipMemsetAsync(d_info, 0, sizeof(int), s));
+    panel_begin<T>(n, s);
     getrf_level<T>(n, A, lda, 0, n, ipiv, d_info, s, variant, 0);
 }
 

@@ -224,6 +224,27 @@ def test_lu_solve_many_rhs(sm, op):
     assert np.linalg.norm(M @ X - b) / (np.linalg.norm(M) * np.linalg.norm(X)) < 1e-14
 
 
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("op", ["N", "C"])
+def test_lu_solve_wide_rhs(sm, prec, op):
+    # more than 4096 right-hand sides (ragged in 64): the one-lane-per-column
+    # small triangular solves, all four triangle forms through ops N and C
+    rng = np.random.default_rng(22)
+    n, nrhs = 100, 4500
+    A = crandn(rng, n, n)
+    b = crandn(rng, n, nrhs)
+    dt = torch.complex128 if prec == 64 else torch.complex64
+    print(f"lu wide rhs prec={prec} op={op}", flush=True)
+    lu = sm.linalg.DenseLuSolver()
+    lu.factorize(to_dev(A, dt))
+    Bd = to_dev(b, dt)
+    lu.solve(Bd, op=op)
+    M = {"N": A, "T": A.T, "C": A.conj().T}[op]
+    X = Bd.cpu().numpy()
+    res = np.linalg.norm(M @ X - b, axis=0) / (np.linalg.norm(M) * np.linalg.norm(X, axis=0))
+    assert res.max() < (1e-14 if prec == 64 else 5e-6), res.max()
+
+
 def _gmres_system(rng, n, shift):
     # complex Gaussian entries: eigenvalues in a disk of radius ~sqrt(2n) about shift sqrt(n)
     A = crandn(rng, n, n) + shift * np.sqrt(n) * np.eye(n)
