@@ -1500,8 +1500,9 @@ __device__ __forceinline__ void trsv_block_solve(const cx<T>* __restrict__ M, co
     const int t = threadIdx.x, i = t % B, part = t / B;
     T ar = 0, ai = 0;
     // no early exit: every load is independent and in bounds (M is a full
-    // B x B block), so the loop pipelines; entries past nw contribute zero
-#pragma unroll 8
+    // B x B block), so all J loads are issued before the first use; entries
+    // past nw contribute zero
+#pragma unroll
     for (int jj = 0; jj < J; ++jj) {
         const int j = part * J + jj;
         const cx<T> x = j < nw ? v[j] : cx<T>{0, 0};
@@ -1549,16 +1550,18 @@ __device__ __forceinline__ void sweep_tile(int op, int w, int k0, const cx<TF>* 
         const int row = t % B, grp = t / B, r = rb + row;
         T ar = 0, ai = 0;
         if (r < r1) {
-            const cx<TF>* a = LU + (long long)(k0 + grp * CG) * lda + r;
+            // every load of the row issued before the first use: columns past
+            // w read column w - 1 (in bounds) and are multiplied by sy = 0
+            const cx<TF>* a = LU + (long long)k0 * lda + r;
+            cx<TF> f[CG];
+#pragma unroll
+            for (int cc = 0; cc < CG; ++cc) f[cc] = a[(long long)min(grp * CG + cc, w - 1) * lda];
 #pragma unroll
             for (int cc = 0; cc < CG; ++cc) {
                 const int c = grp * CG + cc;
-                if (c < w) {
-                    const cx<TF> f = a[(long long)cc * lda];
-                    const cx<T> v = {(T)f.re, (T)f.im};
-                    ar += v.re * sy[c].re - v.im * sy[c].im;
-                    ai += v.re * sy[c].im + v.im * sy[c].re;
-                }
+                const cx<T> v = {(T)f[cc].re, (T)f[cc].im};
+                ar += v.re * sy[c].re - v.im * sy[c].im;
+                ai += v.re * sy[c].im + v.im * sy[c].re;
             }
         }
         spr[grp][row] = ar;
@@ -1578,16 +1581,17 @@ __device__ __forceinline__ void sweep_tile(int op, int w, int k0, const cx<TF>* 
             const int row = pass * RP + rr, r = rb + row;
             T ar = 0, ai = 0;
             if (r < r1) {
-                const cx<TF>* a = LU + (long long)r * lda + k0 + sub * 4;
+                // loads first; columns past w read column w - 1, times sy = 0
+                const cx<TF>* a = LU + (long long)r * lda + k0;
+                cx<TF> f[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) f[e] = a[min(sub * 4 + e, w - 1)];
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     const int c = sub * 4 + e;
-                    if (c < w) {
-                        const cx<TF> f = a[e];
-                        const cx<T> v = {(T)f.re, (T)f.im};
-                        ar += v.re * sy[c].re + v.im * sy[c].im;
-                        ai += v.re * sy[c].im - v.im * sy[c].re;
-                    }
+                    const cx<T> v = {(T)f[e].re, (T)f[e].im};
+                    ar += v.re * sy[c].re + v.im * sy[c].im;
+                    ai += v.re * sy[c].im - v.im * sy[c].re;
                 }
             }
             for (int off = L / 2; off > 0; off >>= 1) {
@@ -1614,7 +1618,7 @@ __device__ __forceinline__ void wait_block_flag(const unsigned int* f) {
 }
 
 // Diagonal blocks solved per launch of the sweep kernel.
-constexpr int kSweepChain = 8;
+constexpr int kSweepChain = 16;
 
 // One launch of a block triangular sweep: with the solution block kb known
 // (in out), solves the next D blocks (kb+1 .. kb+D forward, kb-1 .. kb-D
@@ -2115,12 +2119,14 @@ __global__ void copy_vec_kernel(int n, const cx<T>* __restrict__ in, cx<T>* __re
     if (i < n) out[i] = in[i];
 }
 
-// Diagonal block size of the single-RHS sweeps, by factor type: fewer, wider
-// dependent steps for FP32 (the sweeps are bound by the latency of their ~2n/B
-// launches).
+// Diagonal block size of the single-RHS sweeps. Each dependent step reads a
+// B x B factor tile and a B x B inverse through one workgroup, whose CU
+// streams ~45 GB/s: the critical data per sweep grows as n B while the fixed
+// cost per step (flag, block load, publication: ~2 us) grows as n / B; B = 64
+// balances them (measured against 128 at N = 72k).
 template <typename T>
 struct SweepCfg {
-    static constexpr int B = std::is_same<T, float>::value ? 128 : 64;
+    static constexpr int B = 64;
 };
 
 // Generation of the last factorisation of each array, so that the sweeps can
