@@ -645,9 +645,12 @@ __global__ __launch_bounds__(256) void gather_kernel(
     int nf, int ne, int n0, int nc, const int* __restrict__ elist, int nlist,
     const int* __restrict__ ef, const int* __restrict__ el, const cplx* __restrict__ W,
     const cplx* __restrict__ M /* nf x 144, GSTC only */, Epi ep, cplx* __restrict__ Z,
-    long long ldz, int col_stride) {
+    long long ldz, int col_stride, int overwrite) {
     // col_stride 0: columns of the full matrix (edge id + block E); > 0: a slab
-    // whose block c holds the chunk's edges in list order at c * col_stride
+    // whose block c holds the chunk's edges in list order at c * col_stride.
+    // overwrite: the only chunk (every face pair present), so every entry of
+    // the lane's blocks is stored, zeros included, and Z need not be cleared
+    // first; otherwise the chunk's contributions are added.
     constexpr int NS = NK * 18;
     const int em = blockIdx.x * blockDim.x + threadIdx.x;
     const int li = uniform(blockIdx.y);
@@ -676,6 +679,10 @@ __global__ __launch_bounds__(256) void gather_kernel(
     }
     const long long E = ne;
     auto col = [&](int c) -> long long { return col_stride > 0 ? (long long)li + (long long)c * col_stride : en + c * E; };
+    auto zput = [&](long long r, long long c, cplx v) {
+        if (overwrite) Z[c * ldz + r] = v;
+        else zacc(Z, ldz, r, c, v);
+    };
     if (KIND == K_GSTC) {
         const cplx TL1 = acc[0], TK1 = acc[1];
         const cplx TL2 = NK == 2 ? acc[2] : acc[0], TK2 = NK == 2 ? acc[3] : acc[1];
@@ -722,10 +729,10 @@ __global__ __launch_bounds__(256) void gather_kernel(
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
                 const bool cal = (r >> 1) == (c >> 1);
-                if (cal || shared_g >= 0) zacc(Z, ldz, em + r * E, col(c), blk[r][c]);
+                if (cal || shared_g >= 0 || overwrite) zput(em + r * E, col(c), blk[r][c]);
             }
     } else if (KIND == K_PEC) {
-        zacc(Z, ldz, em, col(0), cmul(ep.c[0], NK == 2 ? acc[2] : acc[0]));
+        zput(em, col(0), cmul(ep.c[0], NK == 2 ? acc[2] : acc[0]));
     } else if (KIND == K_PMCHWT) {
         const cplx TL1 = acc[0], TK1 = acc[1];
         const cplx TL2 = NK == 2 ? acc[2] : acc[0], TK2 = NK == 2 ? acc[3] : acc[1];
@@ -735,10 +742,10 @@ __global__ __launch_bounds__(256) void gather_kernel(
         const cplx d = cmul(ep.c[2], TL1), e = cmul(ep.c[3], TL2);
         const cplx zmm = {d.re + e.re, d.im + e.im};
         const double et = ep.eta, et2 = ep.jm;  // eta0 and (float) eta0^2
-        zacc(Z, ldz, em, col(0), zjj);
-        zacc(Z, ldz, em, col(1), {tk.re * et, tk.im * et});
-        zacc(Z, ldz, em + E, col(0), {-tk.re * et, -tk.im * et});
-        zacc(Z, ldz, em + E, col(1), {zmm.re * et2, zmm.im * et2});
+        zput(em, col(0), zjj);
+        zput(em, col(1), {tk.re * et, tk.im * et});
+        zput(em + E, col(0), {-tk.re * et, -tk.im * et});
+        zput(em + E, col(1), {zmm.re * et2, zmm.im * et2});
     }
 }
 
@@ -1157,9 +1164,11 @@ static void assemble_generic(sie_mesh* h, const sie_calderon_params* p, const Ep
     const size_t wsz = (size_t)nc_max * NK * 18 * m.nf;
     if (A.W.n < wsz) A.W.alloc(wsz);
     const int blocks = KIND == K_GSTC ? 4 : (KIND == K_PMCHWT ? 2 : 1);
+    // one chunk holding every source face: the gather stores every entry of Z
+    const bool single = nc_max >= m.nf;
     if (x) {
         fill_kernel<<<(unsigned)std::min<long long>((dim + 255) / 256, 1 << 16), 256, 0, s>>>(y, dim, dim, 1);
-    } else {
+    } else if (!single) {
         const long long tot = (long long)dim * dim;
         fill_kernel<<<(unsigned)std::min<long long>((tot + 255) / 256, 1 << 16), 256, 0, s>>>(Z, ldz, dim, dim);
     }
@@ -1187,7 +1196,7 @@ static void assemble_generic(sie_mesh* h, const sie_calderon_params* p, const Ep
             fill_kernel<<<(unsigned)std::min<long long>((tot + 255) / 256, 1 << 16), 256, 0, s>>>(A.slab.p, dim, dim,
                                                                                                   (int)cols);
             gather_kernel<NK, KIND><<<grid, 256, 0, s>>>(m.nf, m.ne, n0, nc, A.elist.p, L, m.d_ef.p, m.d_el.p, A.W.p,
-                                                         Mblk, ep, A.slab.p, dim, L);
+                                                         Mblk, ep, A.slab.p, dim, L, 0);
             slab_x_kernel<<<(unsigned)((cols + 255) / 256), 256, 0, s>>>(L, blocks, m.ne, A.elist.p, x, A.xs.p);
             SIE_HIP(hipGetLastError());
             const int rc = sie_zgemv(0, dim, (int)cols, 1.0, (const double*)A.slab.p, dim, (const double*)A.xs.p,
@@ -1195,7 +1204,7 @@ static void assemble_generic(sie_mesh* h, const sie_calderon_params* p, const Ep
             if (rc != SIE_OK) throw Error(rc, sie_last_error());
         } else {
             gather_kernel<NK, KIND><<<grid, 256, 0, s>>>(m.nf, m.ne, n0, nc, A.elist.p, L, m.d_ef.p, m.d_el.p, A.W.p,
-                                                         Mblk, ep, Z, ldz, 0);
+                                                         Mblk, ep, Z, ldz, 0, single ? 1 : 0);
         }
         SIE_HIP(hipGetLastError());
         SIE_HIP(hipStreamSynchronize(s));  // list buffer reused by the next chunk

@@ -304,6 +304,12 @@ __global__ __launch_bounds__(256) void maxabs_kernel(long long rows, long long c
     if (threadIdx.x == 0) atomicMax(out, __float_as_uint(fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]))));
 }
 
+// mx[0] = a bound given by the caller (float bits), mx[1] = 0 for maxabs_kernel.
+__global__ void set_mx_kernel(unsigned int* __restrict__ mx, unsigned int a_bits) {
+    mx[0] = a_bits;
+    mx[1] = 0u;
+}
+
 // 2^-e with max = f 2^e, f in [0.5, 1): scaled entries have modulus < 1.
 __device__ __forceinline__ float pow2_scale(unsigned int bits) {
     const float m = __uint_as_float(bits);
@@ -1762,8 +1768,11 @@ void gemm<double>(int M, int N, int K, double alpha, const c64* A, long long lda
 }
 
 // FP32-complex GEMM on the FP16 matrix cores (split-FP16, see the kernel).
+// a_bound > 0: a bound on max(|re|, |im|) of A known to the caller (the LU's
+// L blocks: |l| <= sqrt 2 under |re| + |im| pivoting), used for A's scale in
+// place of a pass over A.
 static void gemm_f16x3(int M, int N, int K, double alpha, const c32* A, long long lda, const c32* B, long long ldb,
-                       double beta, c32* C, long long ldc, hipStream_t s) {
+                       double beta, c32* C, long long ldc, hipStream_t s, float a_bound = 0.f) {
     if (M <= 0 || N <= 0) return;
     if (K <= 0 || alpha == 0.0) {  // C = beta C
         gemm<float>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s, 0);
@@ -1781,13 +1790,14 @@ static void gemm_f16x3(int M, int N, int K, double alpha, const c32* A, long lon
         require(cap >= 256, "split-FP16 GEMM: K too large");
         if (M > cap) {
             for (long long r0 = 0; r0 < M; r0 += cap)
-                gemm_f16x3((int)std::min<long long>(cap, M - r0), N, K, alpha, A + r0, lda, B, ldb, beta, C + r0, ldc, s);
+                gemm_f16x3((int)std::min<long long>(cap, M - r0), N, K, alpha, A + r0, lda, B, ldb, beta, C + r0, ldc, s,
+                           a_bound);
             return;
         }
         if (N > cap) {
             for (long long c0 = 0; c0 < N; c0 += cap)
                 gemm_f16x3(M, (int)std::min<long long>(cap, N - c0), K, alpha, A, lda, B + c0 * ldb, ldb, beta,
-                           C + c0 * ldc, ldc, s);
+                           C + c0 * ldc, ldc, s, a_bound);
             return;
         }
     }
@@ -1797,9 +1807,13 @@ static void gemm_f16x3(int M, int N, int K, double alpha, const c32* A, long lon
     _Float16* PA = (_Float16*)w;
     _Float16* PB = PA + pa;
     unsigned int* mx = (unsigned int*)(w + (pa + pb) * sizeof(_Float16));
-    SIE_HIP(hipMemsetAsync(mx, 0, 2 * sizeof(unsigned int), s));
     const long long ta = (long long)M * K, tb = (long long)K * N;
-    maxabs_kernel<<<(unsigned)std::min<long long>((ta + 255) / 256, 1024), 256, 0, s>>>(M, K, A, lda, mx);
+    if (a_bound > 0.f) {
+        set_mx_kernel<<<1, 1, 0, s>>>(mx, __builtin_bit_cast(unsigned int, a_bound));
+    } else {
+        SIE_HIP(hipMemsetAsync(mx, 0, 2 * sizeof(unsigned int), s));
+        maxabs_kernel<<<(unsigned)std::min<long long>((ta + 255) / 256, 1024), 256, 0, s>>>(M, K, A, lda, mx);
+    }
     maxabs_kernel<<<(unsigned)std::min<long long>((tb + 255) / 256, 1024), 256, 0, s>>>(K, N, B, ldb, mx + 1);
     if (r3) {
         const long long ra = (long long)((M + 255) / 256) * 256 * KB * 2, rb = (long long)((N + 127) / 128) * 128 * KB * 2;
@@ -1992,7 +2006,7 @@ static void update(int M, int N, int K, const cx<T>* L, const cx<T>* U, cx<T>* C
     if (M <= 0 || N <= 0 || K <= 0) return;
     if constexpr (std::is_same<T, float>::value) {
         if (variant == 2) {
-            gemm_f16x3(M, N, K, -1.0, L, lda, U, lda, 1.0, C, lda, s);
+            gemm_f16x3(M, N, K, -1.0, L, lda, U, lda, 1.0, C, lda, s, 1.5f);
             return;
         }
     }
