@@ -143,6 +143,46 @@ def copy_calibration():
     print(json.dumps({"copy_bytes_read": x.numel() * 4, "copy_bytes_written": y.numel() * 4}))
 
 
+def lu_gemms(n=72048):
+    """The split-FP16 products of the FP32 LU at N = n, on their own, in the
+    factorisation's order and shapes: per 4096-column level (width w, first
+    column c), after NB step s the in-level update C (M x N) -= L (M x 256)
+    U (256 x N) with M = n - c - 256 (s + 1), N = w - 256 (s + 1); then the
+    level's triangular solve for the columns right of the level (its updates:
+    M = w - 256 (s + 1), N = n - c - w, K = 256) and the level's trailing update
+    M = N = n - c - w, K = 4096 (the wide kernel). Seeded data; the workload
+    of the counter passes (HBM bytes per launch depend on the shapes only)."""
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "sie-gstc-2026_amd"))
+    import sie_mom_amd as sm
+    g = torch.Generator(device="cuda").manual_seed(7)
+    Cbuf = torch.randn(n * n, dtype=torch.complex64, device="cuda", generator=g)
+    Lb = torch.randn(n * 4096, dtype=torch.complex64, device="cuda", generator=g) * 0.5
+    Ub = torch.randn(4096 * n, dtype=torch.complex64, device="cuda", generator=g)
+    launches = 0
+
+    def upd(M, N, K):
+        nonlocal launches
+        if M <= 0 or N <= 0:
+            return
+        A = Lb[: M * K].view(K, M).T
+        B = Ub[: N * K].view(N, K).T
+        C = Cbuf[: M * N].view(N, M).T
+        sm.linalg.gemm(A, B, C, alpha=-1.0, beta=1.0, variant="f16x3")
+        launches += 1
+
+    for c in range(0, n, 4096):
+        w = min(4096, n - c)
+        for st in range(w // 256):
+            upd(n - c - 256 * (st + 1), w - 256 * (st + 1), 256)
+        right = n - c - w
+        for st in range(w // 256):
+            upd(w - 256 * (st + 1), right, 256)
+        upd(right, right, 4096)
+    torch.cuda.synchronize()
+    print(json.dumps({"lu_gemm_launches": launches, "n": n}))
+
+
 def _pmc_mean_kib(csv_path, counter, name_part):
     import csv
     per = {}
@@ -172,7 +212,8 @@ def summarize_pmc(fetch_csv, write_csv, cal_fetch_csv, cal_write_csv, command):
             continue
         traffic = (f * ff + w * fw) * 1024.0
         with open(path, "w") as out:
-            out.write(f"# HBM traffic of {name_part} (LU trailing update), gstc-spindle-72k, one iteration\n")
+            out.write(f"# HBM traffic of {name_part} (LU trailing updates at N = 72048: the factorisation's shapes, "
+                      f"run alone by bench.py --lu-gemms)\n")
             out.write(f"# command: {command}\n")
             out.write(f"launches_fetch: {nf}\nlaunches_write: {nw}\n")
             out.write(f"fetch_kib_per_launch_raw: {f:.1f}\nwrite_kib_per_launch_raw: {w:.1f}\n")
@@ -197,6 +238,8 @@ def traffic_per_launch(kernel):
 
 
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--lu-gemms":
+        return lu_gemms()
     if len(sys.argv) > 1 and sys.argv[1] == "--copy-calibration":
         return copy_calibration()
     if len(sys.argv) > 1 and sys.argv[1] == "--summarize-pmc":
