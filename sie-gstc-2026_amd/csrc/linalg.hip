@@ -1085,14 +1085,14 @@ __global__ __launch_bounds__(BS) void panel_getf2_keyed_kernel(int M, int w, cx<
 
 // ============================================================== row swaps
 // Apply ipiv[k] (0-based, relative to row offset r0) for k in [k_begin, k_end)
-// in order to columns [c0, c0+nc) of A. One lane per column keeps the
-// sequence of swaps in order.
+// in order to columns [c0, c0+nc) and [c1, c1+nc1) of A (the second range may
+// be empty). One lane per column keeps the sequence of swaps in order.
 template <typename T>
-__global__ void laswp_kernel(cx<T>* __restrict__ A, long long lda, int c0, int nc, const int* __restrict__ ipiv,
-                             int k_begin, int k_end, int r0, int reverse) {
+__global__ void laswp_kernel(cx<T>* __restrict__ A, long long lda, int c0, int nc, int c1, int nc1,
+                             const int* __restrict__ ipiv, int k_begin, int k_end, int r0, int reverse) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nc) return;
-    cx<T>* col = A + (long long)(c0 + c) * lda + r0;
+    if (c >= nc + nc1) return;
+    cx<T>* col = A + (long long)(c < nc ? c0 + c : c1 + (c - nc)) * lda + r0;
     if (!reverse) {
         for (int k = k_begin; k < k_end; ++k) {
             const int p = ipiv[k];
@@ -1851,7 +1851,19 @@ template <typename T>
 static void laswp(cx<T>* A, long long lda, int c0, int nc, const int* ipiv, int kb, int ke, int r0, bool rev,
                   hipStream_t s) {
     if (nc <= 0 || ke <= kb) return;
-    laswp_kernel<T><<<(nc + 127) / 128, 128, 0, s>>>(A, lda, c0, nc, ipiv, kb, ke, r0, rev ? 1 : 0);
+    laswp_kernel<T><<<(nc + 127) / 128, 128, 0, s>>>(A, lda, c0, nc, 0, 0, ipiv, kb, ke, r0, rev ? 1 : 0);
+    SIE_HIP(hipGetLastError());
+}
+
+// The forward swaps of rows [kb, ke) on two column ranges in one launch (the
+// LU's columns left and right of the block just factored).
+template <typename T>
+static void laswp2(cx<T>* A, long long lda, int c0, int nc, int c1, int nc1, const int* ipiv, int kb, int ke,
+                   hipStream_t s) {
+    nc = std::max(nc, 0);
+    nc1 = std::max(nc1, 0);
+    if (nc + nc1 <= 0 || ke <= kb) return;
+    laswp_kernel<T><<<(nc + nc1 + 127) / 128, 128, 0, s>>>(A, lda, c0, nc, c1, nc1, ipiv, kb, ke, 0, 0);
     SIE_HIP(hipGetLastError());
 }
 
@@ -2066,8 +2078,7 @@ static void getrf_block(int n, cx<T>* A, long long lda, int c0, int c1, int* ipi
             const int w = std::min(W, k0 + nb - k1);
             cx<T>* P = A + (long long)k1 * lda + k1;
             panel<T>(n - k1, w, P, lda, ipiv + k1, k1, d_info, k1, s);
-            laswp<T>(A, lda, k0, k1 - k0, ipiv, k1, k1 + w, 0, false, s);
-            laswp<T>(A, lda, k1 + w, k0 + nb - (k1 + w), ipiv, k1, k1 + w, 0, false, s);
+            laswp2<T>(A, lda, k0, k1 - k0, k1 + w, k0 + nb - (k1 + w), ipiv, k1, k1 + w, s);
             const int rest = k0 + nb - (k1 + w);
             if (rest > 0) {
                 trsm_small<T>(w, P, lda, A + (long long)(k1 + w) * lda + k1, lda, rest, 0, s);
@@ -2077,10 +2088,9 @@ static void getrf_block(int n, cx<T>* A, long long lda, int c0, int c1, int* ipi
             }
         }
         // this NB panel's swaps on the other columns of the block
-        laswp<T>(A, lda, c0, k0 - c0, ipiv, k0, k0 + nb, 0, false, s);
         const int right = c1 - k0 - nb;
+        laswp2<T>(A, lda, c0, k0 - c0, k0 + nb, right, ipiv, k0, k0 + nb, s);
         if (right > 0) {
-            laswp<T>(A, lda, k0 + nb, right, ipiv, k0, k0 + nb, 0, false, s);
             trsm_lower<T>(A, lda, k0, nb, k0 + nb, right, s, variant);
             update<T>(n - k0 - nb, right, nb, A + (long long)k0 * lda + k0 + nb, A + (long long)(k0 + nb) * lda + k0,
                       A + (long long)(k0 + nb) * lda + k0 + nb, lda, s, variant);
@@ -2104,10 +2114,9 @@ static void getrf_level(int n, cx<T>* A, long long lda, int c0, int c1, int* ipi
     for (int K0 = c0; K0 < c1; K0 += bw) {
         const int sp = std::min(bw, c1 - K0);
         getrf_level<T>(n, A, lda, K0, K0 + sp, ipiv, d_info, s, variant, level + 1);
-        laswp<T>(A, lda, c0, K0 - c0, ipiv, K0, K0 + sp, 0, false, s);
         const int right = c1 - K0 - sp;
+        laswp2<T>(A, lda, c0, K0 - c0, K0 + sp, right, ipiv, K0, K0 + sp, s);
         if (right > 0) {
-            laswp<T>(A, lda, K0 + sp, right, ipiv, K0, K0 + sp, 0, false, s);
             trsm_lower<T>(A, lda, K0, sp, K0 + sp, right, s, variant, level + 1);
             update<T>(n - K0 - sp, right, sp, A + (long long)K0 * lda + K0 + sp, A + (long long)(K0 + sp) * lda + K0,
                       A + (long long)(K0 + sp) * lda + K0 + sp, lda, s, variant);
