@@ -1625,6 +1625,9 @@ __device__ __forceinline__ void wait_block_flag(const unsigned int* f) {
 
 // Diagonal blocks solved per launch of the sweep kernel.
 constexpr int kSweepChain = 16;
+// Row blocks a bulk workgroup keeps in LDS across a launch (one workgroup per
+// CU shares ~n / (64 * 240) blocks: 5 at n = 72k).
+constexpr int kSweepBulkRows = 12;
 
 // One launch of a block triangular sweep: with the solution block kb known
 // (in out), solves the next D blocks (kb+1 .. kb+D forward, kb-1 .. kb-D
@@ -1672,21 +1675,36 @@ __global__ __launch_bounds__(kSweepThreads) void trsv_chain_kernel(int n, int di
         if (t == 0) __hip_atomic_store(&flags[rbk], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return;
     }
+    // bulk: the workgroup's row blocks stay in LDS across the D source blocks
+    // (up to kSweepBulkRows blocks; a larger share streams through global)
     const int nbeyond = dir > 0 ? nb - (kb + 1 + D) : kb - D;
+    __shared__ cx<T> sown[kSweepBulkRows][B];
+    int nown = 0;
+    for (int i = wg - D; i < nbeyond && nown < kSweepBulkRows; i += nbulk, ++nown) {
+        const int rb = (dir > 0 ? kb + 1 + D + i : i) * B;
+        if (t < B) sown[nown][t] = rb + t < n ? vec[rb + t] : cx<T>{0, 0};
+    }
     for (int q = 0; q < D; ++q) {
         const int src = dir > 0 ? kb + q : kb - q;
         if (q > 0 && t == 0) wait_block_flag(&flags[src]);
         __syncthreads();
         const int w = min(B, n - src * B);
         if (t < B) sy[t] = t < w ? load_agent_cx(&out[(size_t)src * B + t]) : cx<T>{0, 0};
-        for (int i = wg - D; i < nbeyond; i += nbulk) {
+        int o = 0;
+        for (int i = wg - D; i < nbeyond; i += nbulk, ++o) {
             const int rb = (dir > 0 ? kb + 1 + D + i : i) * B;
-            if (t < B) sres[t] = rb + t < n ? vec[rb + t] : cx<T>{0, 0};
+            cx<T>* res = o < kSweepBulkRows ? sown[o] : sres;
+            if (o >= kSweepBulkRows && t < B) sres[t] = rb + t < n ? vec[rb + t] : cx<T>{0, 0};
             __syncthreads();
-            sweep_tile<TF, T, B>(op, w, src * B, LU, lda, rb, n, sy, sres, spr, spi);
+            sweep_tile<TF, T, B>(op, w, src * B, LU, lda, rb, n, sy, res, spr, spi);
             __syncthreads();
-            if (t < B && rb + t < n) vec[rb + t] = sres[t];
+            if (o >= kSweepBulkRows && t < B && rb + t < n) vec[rb + t] = sres[t];
         }
+    }
+    int o = 0;
+    for (int i = wg - D; i < nbeyond && o < kSweepBulkRows; i += nbulk, ++o) {
+        const int rb = (dir > 0 ? kb + 1 + D + i : i) * B;
+        if (t < B && rb + t < n) vec[rb + t] = sown[o][t];
     }
 }
 
