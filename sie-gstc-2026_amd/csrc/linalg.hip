@@ -1642,7 +1642,7 @@ constexpr int kSweepBulkRows = 12;
 // workgroup is resident. Every row receives the same products in the same
 // order as with one source block per launch.
 template <typename TF, typename T, int B>
-__global__ __launch_bounds__(kSweepThreads) void trsv_chain_kernel(int n, int dir, int kb, int D, int nbulk, int op,
+__global__ __launch_bounds__(kSweepThreads, sizeof(TF) == 4 ? 8 : 4) void trsv_chain_kernel(int n, int dir, int kb, int D, int nbulk, int op,
                                                                    const cx<TF>* __restrict__ LU, long long lda,
                                                                    cx<T>* __restrict__ vec, const cx<T>* __restrict__ inv,
                                                                    cx<T>* __restrict__ out,
@@ -2196,12 +2196,14 @@ static void record_sweeps(int op, int n, const cx<TF>* LU, long long lda, cx<T>*
     const int nb = (n + B - 1) / B;
     const unsigned g = (unsigned)((n + 255) / 256);
     auto width = [&](int k) { return std::min(B, n - k); };
-    // bulk workgroups per launch: one per CU besides the kSweepChain chain
-    // workgroups (one 1024-thread workgroup fills a CU)
+    // bulk workgroups per launch: as many as are resident with the
+    // kSweepChain chain workgroups (FP32 factors: two 1024-thread workgroups
+    // per CU, the kernel being held to 64 registers; FP64 factors: one)
     int dev = 0, ncu = 0;
     SIE_HIP(hipGetDevice(&dev));
     SIE_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    const int bulk_max = std::max(1, ncu - kSweepChain);
+    const int per_cu = sizeof(TF) == 4 ? 2 : 1;
+    const int bulk_max = std::max(1, per_cu * ncu - kSweepChain);
     // the per-block "solved" flags of both sweeps start at zero (a kernel, not
     // a memset node, so that the sequence can be captured and replayed)
     zero_u32_kernel<<<(2 * nb + 255) / 256, 256, 0, s>>>(2 * nb, flags);
