@@ -5,13 +5,15 @@
 //                          threshold (same predicate as the reference kernels,
 //                          sie_kernels_assembly.cuh:193-197) are compacted.
 //   2. far_pairs_kernel  - one lane per test face, one wave per source face:
-//                          the Q x Q double quadrature of the far pairs in a
-//                          moment-factored form (below), writing the raw
-//                          TL/TK 3x3 blocks of each pair to the scratch W.
-//   3. near_pairs_kernel - one lane per near pair: regularised quadrature plus
-//                          the Wilton singular part (dunavant_quadrature.cuh:
-//                          207-324 with is_singular, 769-875), overwriting the
-//                          near slots of W.
+//                          the Q x Q double quadrature of every pair but the
+//                          self pair in a moment-factored form (below), with
+//                          the regularised kernels on near pairs, writing the
+//                          raw TL/TK 3x3 blocks of each pair to the scratch W.
+//   3. near_pairs_kernel - one lane per near pair: the Wilton singular part
+//                          (dunavant_quadrature.cuh:769-875) added to the near
+//                          slots of W; self pairs get their regularised
+//                          quadrature here too (dunavant_quadrature.cuh:207-324
+//                          with is_singular).
 //   4. gather kernels    - one lane per (test edge, source edge): sum the four
 //                          face-pair contributions of the RWG pair in a fixed
 //                          order, apply the formulation's epilogue (and the
@@ -132,7 +134,15 @@ __global__ __launch_bounds__(256) void far_pairs_kernel(
     const int n = n0 + nl;
     if (SYM && (int)blockIdx.x * 64 + 63 <= n) return;  // no test face of this wave lies above n
     const int mm = m < nf ? m : nf - 1;
-    const bool active = (m < nf) && (!SYM || m > n) && !is_near(c, nf, mm, n, thr);
+    // every pair but the self pair; near pairs (the reference's float centroid
+    // predicate) take the regularised kernels G - 1/(4 pi R) and
+    // K - (1/R^3 + k^2/(2R))/(4 pi) (integrandKernelsRegular,
+    // green_functions_utils.cuh:295-361), whose singular parts
+    // near_pairs_kernel adds
+    const bool active = (m < nf) && (!SYM || m > n) && m != n;
+    const bool nearp = active && is_near(c, nf, mm, n, thr);
+    const double rg = nearp ? 1.0 : 0.0;
+    const bool anyreg = __any(nearp);
 
     // test-face geometry (per lane)
     double om[3][3];
@@ -192,6 +202,26 @@ __global__ __launch_bounds__(256) void far_pairs_kernel(
                 Vb[w][0].re += gr * bx; Vb[w][0].im += gi * bx;
                 Vb[w][1].re += gr * by; Vb[w][1].im += gi * by;
                 Vb[w][2].re += gr * bz; Vb[w][2].im += gi * bz;
+            }
+        }
+        if (anyreg) {
+            // near lanes: the static parts of the kernels, w_q/(4 pi R) in G and
+            // w_q (1/R^3 + k^2/(2R))/(4 pi) in g, taken out of the real moments
+            // (a separate pass, so that the loop above stays that of far pairs)
+            for (int q = 0; q < Q; ++q) {
+                const double bx = bqn[3 * q], by = bqn[3 * q + 1], bz = bqn[3 * q + 2];
+                const double Rx = P[0] - bx, Ry = P[1] - by, Rz = P[2] - bz;
+                const double R2 = Rx * Rx + Ry * Ry + Rz * Rz;
+                const double iR = rsqrt_pos(R2);
+                const double gs = rg * swq[q] * iR;
+#pragma unroll
+                for (int w = 0; w < NK; ++w) {
+                    const double gd = gs * (iR * iR + 0.5 * kk[w] * kk[w]);
+                    T0[w].re -= gs;
+                    V0[w].re -= gd;
+                    Tb[w][0].re -= gs * bx; Tb[w][1].re -= gs * by; Tb[w][2].re -= gs * bz;
+                    Vb[w][0].re -= gd * bx; Vb[w][1].re -= gd * by; Vb[w][2].re -= gd * bz;
+                }
             }
         }
 #pragma unroll
@@ -369,8 +399,9 @@ __global__ __launch_bounds__(128) void near_pairs_kernel(
         for (int j = 0; j < 3; ++j)
             for (int i = 0; i < 3; ++i) tl[w][j][i] = tk[w][j][i] = {0, 0};
 
-    // regularised double quadrature (integrandKernelsRegular)
-    for (int p = 0; p < Q; ++p) {
+    // regularised double quadrature (integrandKernelsRegular) of the self pair;
+    // far_pairs_kernel has written the regular part of every other near pair
+    for (int p = 0; p < (self ? Q : 0); ++p) {
         const double xp = qxy[2 * p], yp = qxy[2 * p + 1], wp = qw[p];
         const double zp = 1.0 - xp - yp;
         const V3 rp = {tt[0].x * zp + tt[1].x * xp + tt[2].x * yp, tt[0].y * zp + tt[1].y * xp + tt[2].y * yp,
@@ -465,11 +496,13 @@ __global__ __launch_bounds__(128) void near_pairs_kernel(
         for (int j = 0; j < 3; ++j)
             for (int i = 0; i < 3; ++i) {
                 const double s = asc * lm[j] * ln[i];
-                cplx a = tl[w][j][i], b = tk[w][j][i];
+                cplx* pa = &Wn[(size_t)((w * 2 + 0) * 9 + j * 3 + i) * nf];
+                cplx* pb = &Wn[(size_t)((w * 2 + 1) * 9 + j * 3 + i) * nf];
+                cplx a = self ? tl[w][j][i] : *pa, b = self ? tk[w][j][i] : *pb;
                 a.re += stl[w][j][i] * s;
                 b.re += stk[w][j][i] * s;
-                Wn[(size_t)((w * 2 + 0) * 9 + j * 3 + i) * nf] = a;
-                Wn[(size_t)((w * 2 + 1) * 9 + j * 3 + i) * nf] = b;
+                *pa = a;
+                *pb = b;
             }
 }
 
