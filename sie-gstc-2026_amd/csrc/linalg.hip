@@ -844,34 +844,67 @@ __device__ __forceinline__ unsigned long long pivot_key(float v, int idx) {
 // registers, so a co-resident grid has at most one workgroup per CU anyway).
 constexpr int kPanelKeyedMaxG = 256;
 
-// Agent-scope (device-coherent) accesses of a complex value, 8 bytes at a
-// time: values published inside a launch (pivot rows of the panel, solution
-// blocks of the sweeps) are read by other workgroups, possibly on other XCDs,
-// without an L2 writeback or invalidate.
+// The block triangular sweeps publish their solution in place: the output
+// vector is first filled with kUnsolved words, each solved entry is written
+// with agent-scope (device-coherent) 64-bit stores, which workgroups on other
+// XCDs read without an L2 writeback or invalidate, and a reader polls the
+// entries themselves (an entry is there once none of its 64-bit words is
+// kUnsolved), so the values are their own ready signal and no flag or store
+// fence is needed. kUnsolved is a NaN pattern (sign and every payload bit
+// set) that arithmetic on other operands does not produce; a computed word
+// equal to it is published as the default quiet NaN.
+constexpr unsigned long long kUnsolved = ~0ull;
+
 template <typename T>
-__device__ __forceinline__ void store_agent_cx(cx<T>* p, cx<T> v) {
+__device__ __forceinline__ unsigned long long published_word(unsigned long long w) {
+    constexpr unsigned long long qnan = sizeof(T) == 8 ? 0x7FF8000000000000ull : 0x7FC000007FC00000ull;
+    return w == kUnsolved ? qnan : w;
+}
+
+template <typename T>
+__device__ __forceinline__ void publish_cx(cx<T>* p, cx<T> v) {
     if constexpr (sizeof(T) == 8) {
-        __hip_atomic_store(reinterpret_cast<unsigned long long*>(&p->re), __builtin_bit_cast(unsigned long long, v.re),
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(reinterpret_cast<unsigned long long*>(&p->im), __builtin_bit_cast(unsigned long long, v.im),
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(reinterpret_cast<unsigned long long*>(&p->re),
+                           published_word<T>(__builtin_bit_cast(unsigned long long, v.re)), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(reinterpret_cast<unsigned long long*>(&p->im),
+                           published_word<T>(__builtin_bit_cast(unsigned long long, v.im)), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
     } else {
-        __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), __builtin_bit_cast(unsigned long long, v),
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(reinterpret_cast<unsigned long long*>(p),
+                           published_word<T>(__builtin_bit_cast(unsigned long long, v)), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
     }
 }
+
+// Bounded wait for one published entry. The sweeps wait only for blocks
+// solved earlier in the sweep or by a workgroup of lower index in the same
+// launch (dispatched earlier and waiting only on workgroups of even lower
+// index), so the wait ends; the bound (seconds) only keeps a broken invariant
+// from hanging the device.
 template <typename T>
-__device__ __forceinline__ cx<T> load_agent_cx(const cx<T>* p) {
-    if constexpr (sizeof(T) == 8) {
-        const unsigned long long re = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(&p->re),
-                                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned long long im = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(&p->im),
-                                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        return cx<T>{__builtin_bit_cast(T, re), __builtin_bit_cast(T, im)};
-    } else {
-        return __builtin_bit_cast(cx<T>, __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p),
-                                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+__device__ __forceinline__ cx<T> wait_published(const cx<T>* p) {
+    unsigned long long w0 = kUnsolved, w1 = kUnsolved;
+    for (unsigned it = 0; it < (1u << 22); ++it) {
+        if constexpr (sizeof(T) == 8) {
+            w0 = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(&p->re), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            w1 = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(&p->im), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            w0 = w1 = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (w0 != kUnsolved && w1 != kUnsolved) break;
+        __builtin_amdgcn_s_sleep(2);
     }
+    if constexpr (sizeof(T) == 8) return cx<T>{__builtin_bit_cast(T, w0), __builtin_bit_cast(T, w1)};
+    else return __builtin_bit_cast(cx<T>, w0);
+}
+
+__global__ void fill_unsolved_kernel(long long nwords, unsigned long long* __restrict__ p) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nwords) p[i] = kUnsolved;
 }
 
 // A published row of the FP32 panel: each complex entry as two 64-bit words,
@@ -1497,24 +1530,32 @@ __global__ void diag_inv_kernel(int n, const cx<TF>* __restrict__ LU, long long 
 // thread in flight at once.
 constexpr int kSweepThreads = 1024;
 
+// Loads of trsv_block_apply: thread t holds M[j B + t % B] for its B / P
+// columns j (P = kSweepThreads / B column groups). The block is a full B x B
+// array, so every load is in bounds whatever the block's width.
+template <typename T, int B>
+__device__ __forceinline__ void trsv_block_load(const cx<T>* __restrict__ M, cx<T>* m) {
+    constexpr int P = kSweepThreads / B, J = B / P;
+    const int t = threadIdx.x, i = t % B, part = t / B;
+#pragma unroll
+    for (int jj = 0; jj < J; ++jj) m[jj] = M[(part * J + jj) * B + i];
+}
+
+// out[i] = sum_j M[j B + i] v_j with M's entries from trsv_block_load: M is
+// the inverse (op N) or its conjugate transpose (op C), column-major; entries
+// past nw contribute zero.
 template <typename T, int B, bool AGENT = false>
-__device__ __forceinline__ void trsv_block_solve(const cx<T>* __restrict__ M, const cx<T>* v, int nw, T (*spr)[B],
-                                                 T (*spi)[B], cx<T>* __restrict__ out) {
-    // out[i] = sum_j M[j B + i] v_j: M is the inverse (op N) or its conjugate
-    // transpose (op C), column-major, so each load is B consecutive entries
+__device__ __forceinline__ void trsv_block_apply(const cx<T>* m, const cx<T>* v, int nw, T (*spr)[B], T (*spi)[B],
+                                                 cx<T>* __restrict__ out) {
     constexpr int P = kSweepThreads / B, J = B / P;
     const int t = threadIdx.x, i = t % B, part = t / B;
     T ar = 0, ai = 0;
-    // no early exit: every load is independent and in bounds (M is a full
-    // B x B block), so all J loads are issued before the first use; entries
-    // past nw contribute zero
 #pragma unroll
     for (int jj = 0; jj < J; ++jj) {
         const int j = part * J + jj;
         const cx<T> x = j < nw ? v[j] : cx<T>{0, 0};
-        const cx<T> m = M[j * B + i];
-        ar += m.re * x.re - m.im * x.im;
-        ai += m.re * x.im + m.im * x.re;
+        ar += m[jj].re * x.re - m[jj].im * x.im;
+        ai += m[jj].re * x.im + m[jj].im * x.re;
     }
     spr[part][i] = ar;
     spi[part][i] = ai;
@@ -1523,9 +1564,18 @@ __device__ __forceinline__ void trsv_block_solve(const cx<T>* __restrict__ M, co
         T re = 0, im = 0;
 #pragma unroll
         for (int q = 0; q < P; ++q) { re += spr[q][t]; im += spi[q][t]; }
-        if constexpr (AGENT) store_agent_cx(out + t, cx<T>{re, im});
+        if constexpr (AGENT) publish_cx(out + t, cx<T>{re, im});
         else out[t] = cx<T>{re, im};
     }
+}
+
+template <typename T, int B, bool AGENT = false>
+__device__ __forceinline__ void trsv_block_solve(const cx<T>* __restrict__ M, const cx<T>* v, int nw, T (*spr)[B],
+                                                 T (*spi)[B], cx<T>* __restrict__ out) {
+    // all loads are issued before the first use
+    cx<T> m[B * B / kSweepThreads];
+    trsv_block_load<T, B>(M, m);
+    trsv_block_apply<T, B, AGENT>(m, v, nw, spr, spi, out);
 }
 
 // Mh = M^H for every B x B block: Mh[blk][j B + i] = conj(M[blk][i B + j]).
@@ -1536,6 +1586,101 @@ __global__ void conj_transpose_blocks_kernel(long long total, const cx<T>* __res
         const int r = (int)(e % (B * B)), j = r / B, i = r % B;
         const cx<T> m = M[blk * B * B + (long long)i * B + j];
         Mh[e] = cx<T>{m.re, -m.im};
+    }
+}
+
+// Factor-tile entries per thread of sweep_tile_load / sweep_tile_apply.
+template <int B>
+constexpr int kSweepTileRegs = B * B / kSweepThreads;
+
+// Loads of sweep_tile_apply: the factor tile of rows [rb, rb + B) and source
+// columns [k0, k0 + w), op 0 by columns (thread t: row t % B, columns of
+// group t / B), op 2 by rows of the stored factor (B / 4 lanes per row, 4
+// entries each). Columns past w read column w - 1 (in bounds; multiplied by
+// sy = 0); rows past r1 are not read.
+template <typename TF, int B>
+__device__ __forceinline__ void sweep_tile_load(int op, int w, int k0, const cx<TF>* __restrict__ LU, long long lda,
+                                                int rb, int r1, cx<TF>* f) {
+    constexpr int P = kSweepThreads / B;   // op N: column groups
+    constexpr int CG = B / P;              // columns per group
+    constexpr int L = B / 4;               // op C: lanes per row, 4 entries each
+    constexpr int RP = kSweepThreads / L;  // op C: rows per pass
+    const int t = threadIdx.x;
+    if (op == 0) {
+        const int row = t % B, grp = t / B, r = rb + row;
+        const cx<TF>* a = LU + (long long)k0 * lda + r;
+#pragma unroll
+        for (int cc = 0; cc < CG; ++cc)
+            f[cc] = r < r1 ? a[(long long)min(grp * CG + cc, w - 1) * lda] : cx<TF>{0, 0};
+    } else {
+        const int sub = t % L, rr = t / L;
+#pragma unroll
+        for (int pass = 0; pass < B / RP; ++pass) {
+            const int r = rb + pass * RP + rr;
+            const cx<TF>* a = LU + (long long)r * lda + k0;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) f[pass * 4 + e] = r < r1 ? a[min(sub * 4 + e, w - 1)] : cx<TF>{0, 0};
+        }
+    }
+}
+
+// sres (the workgroup's B rows of the right-hand side, rows [rb, rb + B), in
+// LDS) -= the factor tile f (from sweep_tile_load) times sy: op 0 the tile of
+// L or U itself, op 2 the tile of U^H or L^H (rows of the stored factor,
+// conjugated). Entry e of the tile is f[e * FS] (FS = 1: registers; the
+// thread's column of an LDS array otherwise). Each row's products are summed
+// in a fixed order, then subtracted once.
+template <typename TF, typename T, int B, int FS = 1>
+__device__ __forceinline__ void sweep_tile_apply(int op, int rb, int r1, const cx<TF>* f, const cx<T>* sy,
+                                                 cx<T>* sres, T (*spr)[B], T (*spi)[B]) {
+    constexpr int P = kSweepThreads / B;
+    constexpr int CG = B / P;
+    constexpr int L = B / 4;
+    constexpr int RP = kSweepThreads / L;
+    const int t = threadIdx.x;
+    if (op == 0) {
+        const int row = t % B, grp = t / B;
+        T ar = 0, ai = 0;
+#pragma unroll
+        for (int cc = 0; cc < CG; ++cc) {
+            const int c = grp * CG + cc;
+            const cx<T> v = {(T)f[cc * FS].re, (T)f[cc * FS].im};
+            ar += v.re * sy[c].re - v.im * sy[c].im;
+            ai += v.re * sy[c].im + v.im * sy[c].re;
+        }
+        spr[grp][row] = ar;
+        spi[grp][row] = ai;
+        __syncthreads();
+        if (t < B && rb + t < r1) {
+            T re = 0, im = 0;
+#pragma unroll
+            for (int q = 0; q < P; ++q) { re += spr[q][t]; im += spi[q][t]; }
+            sres[t].re -= re;
+            sres[t].im -= im;
+        }
+    } else {
+        const int sub = t % L, rr = t / L;
+#pragma unroll
+        for (int pass = 0; pass < B / RP; ++pass) {
+            const int row = pass * RP + rr;
+            T ar = 0, ai = 0;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int c = sub * 4 + e;
+                const cx<TF> fe = f[(pass * 4 + e) * FS];
+                const cx<T> v = {(T)fe.re, (T)fe.im};
+                ar += v.re * sy[c].re + v.im * sy[c].im;
+                ai += v.re * sy[c].im - v.im * sy[c].re;
+            }
+            for (int off = L / 2; off > 0; off >>= 1) {
+                ar += __shfl_xor(ar, off, L);
+                ai += __shfl_xor(ai, off, L);
+            }
+            if (sub == 0 && rb + row < r1) {
+                sres[row].re -= ar;
+                sres[row].im -= ai;
+            }
+        }
     }
 }
 
@@ -1612,17 +1757,6 @@ __device__ __forceinline__ void sweep_tile(int op, int w, int k0, const cx<TF>* 
     }
 }
 
-// Bounded wait for a block of the solution published in the same launch by a
-// workgroup of lower index: that workgroup was dispatched earlier and waits
-// only on workgroups of even lower index, so the wait ends; the bound (about
-// seconds) only keeps a broken invariant from hanging the device.
-__device__ __forceinline__ void wait_block_flag(const unsigned int* f) {
-    for (unsigned it = 0; it < (1u << 22); ++it) {
-        if (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
-        __builtin_amdgcn_s_sleep(2);
-    }
-}
-
 // Diagonal blocks solved per launch of the sweep kernel.
 constexpr int kSweepChain = 16;
 // Row blocks a bulk workgroup keeps in LDS across a launch (one workgroup per
@@ -1645,34 +1779,50 @@ template <typename TF, typename T, int B>
 __global__ __launch_bounds__(kSweepThreads, sizeof(TF) == 4 ? 8 : 4) void trsv_chain_kernel(int n, int dir, int kb, int D, int nbulk, int op,
                                                                    const cx<TF>* __restrict__ LU, long long lda,
                                                                    cx<T>* __restrict__ vec, const cx<T>* __restrict__ inv,
-                                                                   cx<T>* __restrict__ out,
-                                                                   unsigned int* __restrict__ flags) {
+                                                                   cx<T>* __restrict__ out) {
     constexpr int P = kSweepThreads / B;
     __shared__ cx<T> sy[B];
     __shared__ cx<T> sres[B];
     __shared__ T spr[P][B], spi[P][B];
+    __shared__ cx<TF> stile[kSweepTileRegs<B>][kSweepThreads];
     const int t = threadIdx.x;
     const int wg = blockIdx.x;
     const int nb = (n + B - 1) / B;
     if (wg < D) {
         const int rbk = dir > 0 ? kb + 1 + wg : kb - 1 - wg;
         const int rb = rbk * B;
+        // the last source block's factor tile and this block's inverse do not
+        // depend on the solution: they are loaded before the wait for the last
+        // source block, so that once it is published only products remain.
+        // The tile is loaded first and parked in LDS (each thread reads back
+        // its own entries); the inverse is loaded into registers after the
+        // other source blocks, whose tiles need the registers.
+        const int last = dir > 0 ? kb + wg : kb - wg;
+        const int wl = min(B, n - last * B);
+        {
+            cx<TF> f[kSweepTileRegs<B>];
+            sweep_tile_load<TF, B>(op, wl, last * B, LU, lda, rb, n, f);
+#pragma unroll
+            for (int e = 0; e < kSweepTileRegs<B>; ++e) stile[e][t] = f[e];
+        }
         if (t < B) sres[t] = rb + t < n ? vec[rb + t] : cx<T>{0, 0};
-        for (int q = 0; q <= wg; ++q) {
+        for (int q = 0; q < wg; ++q) {
             const int src = dir > 0 ? kb + q : kb - q;
-            if (q > 0 && t == 0) wait_block_flag(&flags[src]);
             __syncthreads();
             const int w = min(B, n - src * B);
-            if (t < B) sy[t] = t < w ? load_agent_cx(&out[(size_t)src * B + t]) : cx<T>{0, 0};
+            if (t < B) sy[t] = t < w ? wait_published(&out[(size_t)src * B + t]) : cx<T>{0, 0};
             __syncthreads();
             sweep_tile<TF, T, B>(op, w, src * B, LU, lda, rb, n, sy, sres, spr, spi);
         }
+        cx<T> m[B * B / kSweepThreads];
+        trsv_block_load<T, B>(inv + (size_t)rbk * B * B, m);
+        __syncthreads();
+        if (t < B) sy[t] = t < wl ? wait_published(&out[(size_t)last * B + t]) : cx<T>{0, 0};
+        __syncthreads();
+        sweep_tile_apply<TF, T, B, kSweepThreads>(op, rb, n, &stile[0][t], sy, sres, spr, spi);
         __syncthreads();
         const int nw = min(B, n - rb);
-        trsv_block_solve<T, B, true>(inv + (size_t)rbk * B * B, sres, nw, spr, spi, out + rb);
-        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        __syncthreads();
-        if (t == 0) __hip_atomic_store(&flags[rbk], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        trsv_block_apply<T, B, true>(m, sres, nw, spr, spi, out + rb);
         return;
     }
     // bulk: the workgroup's row blocks stay in LDS across the D source blocks
@@ -1686,10 +1836,9 @@ __global__ __launch_bounds__(kSweepThreads, sizeof(TF) == 4 ? 8 : 4) void trsv_c
     }
     for (int q = 0; q < D; ++q) {
         const int src = dir > 0 ? kb + q : kb - q;
-        if (q > 0 && t == 0) wait_block_flag(&flags[src]);
         __syncthreads();
         const int w = min(B, n - src * B);
-        if (t < B) sy[t] = t < w ? load_agent_cx(&out[(size_t)src * B + t]) : cx<T>{0, 0};
+        if (t < B) sy[t] = t < w ? wait_published(&out[(size_t)src * B + t]) : cx<T>{0, 0};
         int o = 0;
         for (int i = wg - D; i < nbeyond; i += nbulk, ++o) {
             const int rb = (dir > 0 ? kb + 1 + D + i : i) * B;
@@ -1708,18 +1857,14 @@ __global__ __launch_bounds__(kSweepThreads, sizeof(TF) == 4 ? 8 : 4) void trsv_c
     }
 }
 
-__global__ void zero_u32_kernel(int n, unsigned int* __restrict__ p) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0u;
-}
-
-// out[k0 + i] = M vec[k0 + i] for one diagonal block (the first of a sweep).
+// out[k0 + i] = M vec[k0 + i] for one diagonal block (the first of a sweep),
+// published as the sweep kernels publish theirs.
 template <typename T, int B>
 __global__ __launch_bounds__(kSweepThreads) void trsv_diag_kernel(int k0, int w, const cx<T>* __restrict__ inv, int op,
                                                                   const cx<T>* __restrict__ vec,
                                                                   cx<T>* __restrict__ out) {
     __shared__ T spr[kSweepThreads / B][B], spi[kSweepThreads / B][B];
-    trsv_block_solve<T, B>(inv, vec + k0, w, spr, spi, out + k0);
+    trsv_block_solve<T, B, true>(inv, vec + k0, w, spr, spi, out + k0);
 }
 
 // out[i] = in[perm[i]] (gather) or out[perm[i]] = in[i] (scatter)
@@ -2190,8 +2335,7 @@ static void note_factorisation(const void* A) {
 // invL / invU: diagonal-block inverses for op N, their conjugate transposes for op C.
 template <typename TF, typename T>
 static void record_sweeps(int op, int n, const cx<TF>* LU, long long lda, cx<T>* b, const cx<T>* invL,
-                          const cx<T>* invU, cx<T>* y, cx<T>* t, const int* perm, unsigned int* flags,
-                          hipStream_t s) {
+                          const cx<T>* invU, cx<T>* y, cx<T>* t, const int* perm, hipStream_t s) {
     constexpr int B = SweepCfg<TF>::B;
     const int nb = (n + B - 1) / B;
     const unsigned g = (unsigned)((n + 255) / 256);
@@ -2204,29 +2348,34 @@ static void record_sweeps(int op, int n, const cx<TF>* LU, long long lda, cx<T>*
     SIE_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     const int per_cu = sizeof(TF) == 4 ? 2 : 1;
     const int bulk_max = std::max(1, per_cu * ncu - kSweepChain);
-    // the per-block "solved" flags of both sweeps start at zero (a kernel, not
-    // a memset node, so that the sequence can be captured and replayed)
-    zero_u32_kernel<<<(2 * nb + 255) / 256, 256, 0, s>>>(2 * nb, flags);
+    // each sweep's output starts as kUnsolved words (a kernel, not a memset
+    // node, so that the sequence can be captured and replayed)
+    const long long nwords = (long long)n * (long long)(sizeof(cx<T>) / 8);
+    auto unsolved = [&](cx<T>* out) {
+        fill_unsolved_kernel<<<(unsigned)((nwords + 255) / 256), 256, 0, s>>>(nwords, (unsigned long long*)out);
+    };
     // forward sweep over blocks 0, 1, ...: solve block 0, then kSweepChain blocks per launch
     auto forward = [&](const cx<T>* inv, int sop, cx<T>* vec, cx<T>* out) {
+        unsolved(out);
         trsv_diag_kernel<T, B><<<1, kSweepThreads, 0, s>>>(0, width(0), inv, sop, vec, out);
         for (int kb = 0; kb < nb - 1; kb += kSweepChain) {
             const int D = std::min(kSweepChain, nb - 1 - kb);
             const int nbulk = std::min(std::max(0, nb - (kb + 1 + D)), bulk_max);
             trsv_chain_kernel<TF, T, B><<<D + nbulk, kSweepThreads, 0, s>>>(n, 1, kb, D, nbulk, sop, LU, lda, vec,
-                                                                           inv, out, flags);
+                                                                           inv, out);
         }
     };
     // backward sweep over blocks nb-1, ..., 0
     auto backward = [&](const cx<T>* inv, int sop, cx<T>* vec, cx<T>* out) {
         const int kl = (nb - 1) * B;
+        unsolved(out);
         trsv_diag_kernel<T, B><<<1, kSweepThreads, 0, s>>>(kl, width(kl), inv + (size_t)(nb - 1) * B * B, sop, vec,
                                                             out);
         for (int kb = nb - 1; kb > 0; kb -= kSweepChain) {
             const int D = std::min(kSweepChain, kb);
             const int nbulk = std::min(std::max(0, kb - D), bulk_max);
             trsv_chain_kernel<TF, T, B><<<D + nbulk, kSweepThreads, 0, s>>>(n, -1, kb, D, nbulk, sop, LU, lda, vec,
-                                                                           inv, out, flags + nb);
+                                                                           inv, out);
         }
     };
     if (op == 0) {
@@ -2277,8 +2426,7 @@ void getrs_vec(int op, int n, const cx<TF>* LU, long long lda, const int* ipiv, 
     constexpr int tkey = (int)(sizeof(TF) * 100 + sizeof(T));  // cache key of the type pair
     const int nb = (n + B - 1) / B;
     const size_t invsz = (size_t)nb * B * B * sizeof(cx<T>);
-    unsigned char* w = (unsigned char*)ws(5).get(4 * invsz + 2 * (size_t)n * sizeof(cx<T>) + (size_t)n * sizeof(int) +
-                                                 2 * (size_t)nb * sizeof(unsigned int) + 16);
+    unsigned char* w = (unsigned char*)ws(5).get(4 * invsz + 2 * (size_t)n * sizeof(cx<T>) + (size_t)n * sizeof(int) + 16);
     cx<T>* invL = (cx<T>*)w;
     cx<T>* invU = (cx<T>*)(w + invsz);
     cx<T>* invLh = (cx<T>*)(w + 2 * invsz);
@@ -2286,7 +2434,6 @@ void getrs_vec(int op, int n, const cx<TF>* LU, long long lda, const int* ipiv, 
     cx<T>* y = (cx<T>*)(w + 4 * invsz);
     cx<T>* t = y + n;
     int* perm = (int*)(t + n);
-    unsigned int* flags = (unsigned int*)(perm + n);  // 2 nb sweep flags
     // the row permutation of P^T, composed on the host from the swap sequence
     std::vector<int> hp(n), hpiv(n);
     SIE_HIP(hipMemcpyAsync(hpiv.data(), ipiv, sizeof(int) * n, hipMemcpyDeviceToHost, s));
@@ -2318,7 +2465,7 @@ void getrs_vec(int op, int n, const cx<TF>* LU, long long lda, const int* ipiv, 
     const cx<T>* iL = op == 0 ? invL : invLh;
     const cx<T>* iU = op == 0 ? invU : invUh;
     if (nb < 8) {  // short sequences: launch directly
-        record_sweeps<TF, T>(op, n, LU, lda, b, iL, iU, y, t, perm, flags, s);
+        record_sweeps<TF, T>(op, n, LU, lda, b, iL, iU, y, t, perm, s);
         return;
     }
     hipGraphExec_t exec = nullptr;
@@ -2339,7 +2486,7 @@ void getrs_vec(int op, int n, const cx<TF>* LU, long long lda, const int* ipiv, 
         if (!cache.capture) SIE_HIP(hipStreamCreateWithFlags(&cache.capture, hipStreamNonBlocking));
         hipGraph_t graph = nullptr;
         SIE_HIP(hipStreamBeginCapture(cache.capture, hipStreamCaptureModeThreadLocal));
-        record_sweeps<TF, T>(op, n, LU, lda, b, iL, iU, y, t, perm, flags, cache.capture);
+        record_sweeps<TF, T>(op, n, LU, lda, b, iL, iU, y, t, perm, cache.capture);
         SIE_HIP(hipStreamEndCapture(cache.capture, &graph));
         SIE_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
         SIE_HIP(hipGraphDestroy(graph));
