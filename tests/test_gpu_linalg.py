@@ -38,6 +38,23 @@ def test_gemm(sm, prec, variant, m, n, k):
     assert err < (1e-15 if prec == 64 else 2e-7), err
 
 
+@pytest.mark.parametrize("prec,variant", [(64, "4m"), (32, "4m"), (32, "3m"), (32, "f16x3")])
+@pytest.mark.parametrize("alpha,beta", [(0.5, 2.0), (-3.0, 0.25), (2.0, 0.0), (0.0, 1.5)])
+@pytest.mark.parametrize("m,n,k", [(130, 70, 16), (300, 200, 70)])
+def test_gemm_alpha_beta(sm, prec, variant, alpha, beta, m, n, k):
+    # general scalars: C = beta C + alpha A B, including alpha = 0 and beta = 0
+    rng = np.random.default_rng(m + n + k + 7)
+    A, B, C = crandn(rng, m, k), crandn(rng, k, n), crandn(rng, m, n)
+    dt = torch.complex128 if prec == 64 else torch.complex64
+    Cd = to_dev(C, dt)
+    print(f"gemm prec={prec} {variant} alpha={alpha} beta={beta} m={m} n={n} k={k}", flush=True)
+    sm.linalg.gemm(to_dev(A, dt), to_dev(B, dt), Cd, alpha=alpha, beta=beta, variant=variant)
+    ref = beta * C + alpha * (A @ B)
+    scale = abs(alpha) * np.abs(A).max() * np.abs(B).max() * k + abs(beta) * np.abs(C).max()
+    err = np.abs(Cd.cpu().numpy() - ref).max() / scale
+    assert err < (1e-15 if prec == 64 else 4e-7), err
+
+
 def well_conditioned(rng, n):
     return crandn(rng, n, n) + 2 * np.sqrt(n) * np.eye(n)
 
