@@ -2305,13 +2305,85 @@ static void getrf_level(int n, cx<T>* A, long long lda, int c0, int c1, int* ipi
         const int sp = std::min(bw, c1 - K0);
         getrf_level<T>(n, A, lda, K0, K0 + sp, ipiv, d_info, s, variant, level + 1);
         const int right = c1 - K0 - sp;
-        laswp2<T>(A, lda, c0, K0 - c0, K0 + sp, right, ipiv, K0, K0 + sp, s);
+        // the outermost level's swaps reach the columns to its left only after
+        // the factorisation (apply_deferred_swaps)
+        laswp2<T>(A, lda, c0, level == 0 ? 0 : K0 - c0, K0 + sp, right, ipiv, K0, K0 + sp, s);
         if (right > 0) {
             trsm_lower<T>(A, lda, K0, sp, K0 + sp, right, s, variant, level + 1);
             update<T>(n - K0 - sp, right, sp, A + (long long)K0 * lda + K0 + sp, A + (long long)(K0 + sp) * lda + K0,
                       A + (long long)(K0 + sp) * lda + K0 + sp, lda, s, variant);
         }
     }
+}
+
+// tmp[col][r] = A[c0 + col][src[r]] for r < m (a row gather of nc columns;
+// blockIdx.x runs over rows, so one column's gathers are in flight together).
+template <typename T>
+__global__ void gather_rows_kernel(const cx<T>* __restrict__ A, long long lda, int c0, int m,
+                                   const int* __restrict__ src, cx<T>* __restrict__ tmp) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const long long col = blockIdx.y;
+    if (r < m) tmp[col * m + r] = A[(c0 + col) * lda + src[r]];
+}
+
+// A[c0 + col][a + r] = tmp[col][r] for r < m.
+template <typename T>
+__global__ void put_rows_kernel(cx<T>* __restrict__ A, long long lda, int c0, int a, int m,
+                                const cx<T>* __restrict__ tmp) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const long long col = blockIdx.y;
+    if (r < m) A[(c0 + col) * lda + a + r] = tmp[col * m + r];
+}
+
+// The outermost level's row swaps on the columns to its left, applied once
+// after the factorisation: the columns of block L (kLevels[0] wide) receive
+// the swaps of every later block as one row permutation of rows [end of
+// block L, n), composed on the host from ipiv, gathered into a temporary (at
+// most kDeferredTmpBytes of columns at a time) and copied back. Rows are only
+// moved, so the factors are those of applying each block's swaps as it is
+// factored; the 4096 scattered swaps per column and block become one
+// coalesced pass per column.
+constexpr size_t kDeferredTmpBytes = size_t(1) << 30;
+
+template <typename T>
+static void apply_deferred_swaps(int n, cx<T>* A, long long lda, const int* ipiv, hipStream_t s) {
+    const int bw = kLevels[0];
+    if (n <= bw) return;
+    std::vector<int> hpiv(n);
+    SIE_HIP(hipMemcpyAsync(hpiv.data(), ipiv, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    SIE_HIP(hipStreamSynchronize(s));
+    // src rows of every block, concatenated: block L's run covers rows [a_L, n)
+    std::vector<int> src;
+    std::vector<size_t> off;
+    for (int c = 0; c + bw < n; c += bw) {
+        const int a = c + bw, m = n - a;
+        off.push_back(src.size());
+        src.resize(src.size() + m);
+        int* p = src.data() + off.back();
+        for (int r = 0; r < m; ++r) p[r] = a + r;
+        for (int k = a; k < n; ++k) {
+            const int q = hpiv[k];
+            require(q >= k && q < n, "internal: pivot row outside the trailing rows");
+            if (q != k) std::swap(p[k - a], p[q - a]);
+        }
+    }
+    const size_t src_bytes = (src.size() * sizeof(int) + 255) / 256 * 256;
+    unsigned char* w = (unsigned char*)ws(7).get(src_bytes + kDeferredTmpBytes);
+    int* dsrc = (int*)w;
+    cx<T>* tmp = (cx<T>*)(w + src_bytes);
+    SIE_HIP(hipMemcpyAsync(dsrc, src.data(), src.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    SIE_HIP(hipStreamSynchronize(s));
+    for (size_t L = 0; L < off.size(); ++L) {
+        const int c = (int)L * bw, a = c + bw, m = n - a;
+        const int chunk = (int)std::max<size_t>(1, std::min<size_t>(bw, kDeferredTmpBytes / ((size_t)m * sizeof(cx<T>))));
+        for (int c0 = c; c0 < a; c0 += chunk) {
+            const int nc = std::min(chunk, a - c0);
+            const dim3 g((unsigned)((m + 255) / 256), (unsigned)nc);
+            gather_rows_kernel<T><<<g, 256, 0, s>>>(A, lda, c0, m, dsrc + off[L], tmp);
+            put_rows_kernel<T><<<g, 256, 0, s>>>(A, lda, c0, a, m, tmp);
+        }
+    }
+    SIE_HIP(hipGetLastError());
 }
 
 // Blocked right-looking LU with partial pivoting; ipiv is 0-based absolute.
@@ -2324,6 +2396,7 @@ void getrf(int n, cx<T>* A, long long lda, int* ipiv, int* d_info, hipStream_t s
     SIE_HIP(hipMemsetAsync(d_info, 0, sizeof(int), s));
     panel_begin<T>(n, s);
     getrf_level<T>(n, A, lda, 0, n, ipiv, d_info, s, variant, 0);
+    apply_deferred_swaps<T>(n, A, lda, ipiv, s);
 }
 
 template <typename T>
