@@ -109,26 +109,17 @@ __global__ __launch_bounds__(256, 2) void cgemm_f32_kernel(int M, int N, int K, 
 
     const int nk = (K + G32_KB - 1) / G32_KB;
     gload(0);
-    // C enters the accumulators as (beta / alpha) C, loaded together with the
-    // first operand tiles, so that the epilogue (alpha times the accumulators)
-    // waits for no second memory round trip; exact for the factorisation's
-    // alpha = -1, beta = 1
-    const bool cin = TI == 1 && alpha != 0.f && beta != 0.f;
-    if (cin) {
-        const float sc = beta / alpha;
+    // short K (TI = 1): C is loaded together with the first operand tiles, so
+    // that the epilogue waits for no second memory round trip (the products
+    // are still summed from zero and added to beta C once, as for long K)
+    c32 cpre[TI == 1 ? 16 : 1];
+    if constexpr (TI == 1) {
 #pragma unroll
-        for (int ti = 0; ti < TI; ++ti)
-#pragma unroll
-            for (int tj = 0; tj < TI; ++tj)
-#pragma unroll
-                for (int v = 0; v < 16; ++v) {
-                    const long long i = i0 + wm * 32 * TI + ti * 32 + (lane & 31);
-                    const long long j = j0 + wn * 32 * TI + tj * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-                    const c32 c = i < M && j < N ? C[j * ldc + i] : c32{0.f, 0.f};
-                    acc[0][ti][tj][v] = sc * c.re;
-                    if (MODE == 0) acc[1][ti][tj][v] = sc * c.im;
-                    else acc[2][ti][tj][v] = sc * (c.re + c.im);
-                }
+        for (int v = 0; v < 16; ++v) {
+            const long long i = i0 + wm * 32 + (lane & 31);
+            const long long j = j0 + wn * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+            cpre[v] = beta != 0.f && i < M && j < N ? C[j * ldc + i] : c32{0.f, 0.f};
+        }
     }
     lstore(0);
     __syncthreads();
@@ -186,14 +177,10 @@ __global__ __launch_bounds__(256, 2) void cgemm_f32_kernel(int M, int N, int K, 
                     }
                     c32* p = C + j * ldc + i;
                     c32 c;
-                    if (cin) {
-                        c.re = alpha * re;
-                        c.im = alpha * im;
-                    } else {
-                        c = beta != 0.f ? *p : c32{0.f, 0.f};
-                        c.re = beta * c.re + alpha * re;
-                        c.im = beta * c.im + alpha * im;
-                    }
+                    if constexpr (TI == 1) c = cpre[v];
+                    else c = beta != 0.f ? *p : c32{0.f, 0.f};
+                    c.re = beta * c.re + alpha * re;
+                    c.im = beta * c.im + alpha * im;
                     *p = c;
                 }
             }
