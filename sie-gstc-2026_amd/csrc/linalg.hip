@@ -1782,13 +1782,13 @@ constexpr int kSweepBulkRows = 12;
 // backward) and applies source blocks kb .. kb+-(D-1) to every row block
 // beyond them. Workgroup d < D owns block kb+-(1+d): it applies source blocks
 // kb .. kb+-d in order (waiting for each block solved in this launch), then
-// solves its block with the diagonal-block inverse and publishes it
-// (agent-scope stores, completed before its flag is set). The nbulk
-// workgroups >= D share the row blocks beyond (block i to workgroup D + i mod
-// nbulk) and apply the D source blocks one after the other, each as soon as
-// it is published; nbulk is at most the number of CUs minus D, so that every
-// workgroup is resident. Every row receives the same products in the same
-// order as with one source block per launch.
+// solves its block with the diagonal-block inverse and publishes it in place
+// in out (publish_cx; readers poll the entries). The nbulk workgroups >= D
+// share the row blocks beyond (block i to workgroup D + i mod nbulk) and apply
+// the D source blocks one after the other, each as soon as it is published;
+// nbulk is at most the number of resident workgroups minus D. Every row
+// receives the same products in the same order as with one source block per
+// launch.
 template <typename TF, typename T, int B>
 __global__ __launch_bounds__(kSweepThreads, sizeof(TF) == 4 ? 8 : 4) void trsv_chain_kernel(int n, int dir, int kb, int D, int nbulk, int op,
                                                                    const cx<TF>* __restrict__ LU, long long lda,
