@@ -2392,11 +2392,11 @@ __global__ void copy_vec_kernel(int n, const cx<T>* __restrict__ in, cx<T>* __re
     if (i < n) out[i] = in[i];
 }
 
-// Diagonal block size of the single-RHS sweeps. Each dependent step reads a
-// B x B factor tile and a B x B inverse through one workgroup, whose CU
-// streams ~45 GB/s: the critical data per sweep grows as n B while the fixed
-// cost per step (flag, block load, publication: ~2 us) grows as n / B; B = 64
-// balances them (measured against 128 at N = 72k).
+// Diagonal block size of the single-RHS sweeps. Each dependent step needs a
+// B x B factor tile and a B x B inverse in one workgroup, whose CU streams
+// ~45 GB/s: the data per sweep through that path grows as n B while the fixed
+// cost per step (waiting for the block, reading it, publishing: ~2 us) grows
+// as n / B; B = 64 balances them (measured against 128 at N = 72k).
 template <typename T>
 struct SweepCfg {
     static constexpr int B = 64;
