@@ -9,10 +9,11 @@ default), objective = far-field power in the 10-degree cone about +z.
 
 One step = one synthesis iteration: surface maps from the parameters ->
 FP64 eta0-scaled assembly -> complex-FP32 LU on the matrix cores -> FP64
-iterative refinement -> objective -> adjoint solve with Z^H (the same FP32
-factors + FP64 refinement) -> per-face gradient contraction -> parameter
-update. The plain FP64 path (FP64 LU, FP64 solves) runs once afterwards for
-the speed ratio and the error of the mixed path against it.
+GMRES-IR to a normwise backward error of IR_TOL -> objective -> adjoint
+solve with Z^H (the same FP32 factors + FP64 refinement) -> per-face
+gradient contraction -> parameter update. The plain FP64 path (FP64 LU,
+FP64 solves) then runs the last iterate once untimed (its solution is the
+reference for the errors) and --fp64-steps times timed, for the speed ratio.
 
 Prints one JSON line (rank 0). value = seconds per iteration (lower is better).
 """
@@ -28,6 +29,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "sie-gstc-2026_amd"))
 
 METRIC = "assemble+solve+adjoint-grad s/iter @N≈72k, FP32-MFMA+IR vs FP64, err≤1e-8"
+# refinement stop of both solves: normwise backward error |r| / (|Z|_F |x| + |b|)
+IR_TOL = 1e-15
 
 WORKLOADS = {
     # name: (mesh kind, size parameter, quadrature rule, parameters per face)
@@ -38,13 +41,20 @@ WORKLOADS = {
     "gstc-spindle-8k": ("spindle", 30, 7, 10),
 }
 
-PROF = {"gemm_f32": 0, "gemm_f64": 1, "far_pairs": 2, "panel": 3, "gemm_f16x3": 7, "gemm_f16x3_wide": 8}
-TIMED_PROF = ("gemm_f16x3_wide", "far_pairs")
-UNTIMED_PROF = ("gemm_f32", "panel", "gemm_f16x3")
+PROF = {"gemm_f32": 0, "gemm_f64": 1, "far_pairs": 2, "panel": 3, "near_pairs": 4, "gather": 5, "gemv": 6,
+        "gemm_f16x3": 7, "gemm_f16x3_wide": 8, "sweeps": 9, "contraction": 10, "matvec_free": 11}
+# events inside the timed iterations: only kernels with few launches per iteration
+TIMED_PROF = ("gemm_f16x3_wide", "far_pairs", "near_pairs", "gather", "contraction")
+# the many-launch kernels: events on an untimed re-solve of the last iterate
+UNTIMED_PROF = ("gemm_f32", "panel", "gemm_f16x3", "gemv", "sweeps")
 # Published dense peaks. The split-FP16 GEMM issues three FP16 MFMA products per
 # algorithmic FP32 product, so its ceiling for algorithmic flops is 2.5 PF / 3.
+# "latency": no throughput bound applies (reported as time per unit).
+HBM = 8.0e12
 PEAK = {"gemm_f32": (157.3e12, "mfma"), "gemm_f64": (78.6e12, "mfma"), "far_pairs": (78.6e12, "valu-fp64"),
-        "panel": (8.0e12, "hbm"), "gemm_f16x3": (2.5e15 / 3, "mfma"), "gemm_f16x3_wide": (2.5e15 / 3, "mfma")}
+        "gemm_f16x3": (2.5e15 / 3, "mfma"), "gemm_f16x3_wide": (2.5e15 / 3, "mfma"),
+        "gemv": (HBM, "hbm"), "sweeps": (HBM, "hbm"), "contraction": (HBM, "hbm"), "gather": (HBM, "hbm"),
+        "matvec_free": (78.6e12, "valu-fp64"), "panel": (None, "latency"), "near_pairs": (None, "latency")}
 
 
 DTYPE = {"f16x3": "c64 LU on split-fp16 MFMA (fp32-accurate) + c128 refinement",
@@ -225,6 +235,31 @@ def summarize_pmc(fetch_csv, write_csv, cal_fetch_csv, cal_write_csv, command):
         print(open(path).read())
 
 
+def summarize_stats(stats_csv, out_path, title, command):
+    """Writes a kernel-statistics summary (rocprofv3 --kernel-trace --stats CSV):
+    calls, total and average duration per kernel, by total time."""
+    import csv
+    rows = list(csv.DictReader(open(stats_csv)))
+    tot = sum(float(r["TotalDurationNs"]) for r in rows)
+    with open(out_path, "w") as out:
+        out.write(f"# rocprofv3 kernel statistics: {title}\n# command: {command}\n\n")
+        out.write(f"{'kernel':66s}{'calls':>7s}{'total ms':>11s}{'avg us':>12s}{'%':>7s}\n")
+        for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"])):
+            name = r["Name"]
+            if name.startswith("_Z"):
+                import subprocess
+                try:
+                    name = subprocess.run(["c++filt", name], capture_output=True, text=True).stdout.strip() or name
+                except OSError:
+                    pass
+            name = name.replace("sie::(anonymous namespace)::", "").split("(")[0]
+            t = float(r["TotalDurationNs"])
+            out.write(f"{name[:65]:66s}{int(r['Calls']):7d}{t / 1e6:11.1f}{float(r['AverageNs']) / 1e3:12.1f}"
+                      f"{100 * t / tot:7.2f}\n")
+        out.write(f"# total kernel time {tot / 1e6:.1f} ms\n")
+    print(open(out_path).read())
+
+
 def traffic_per_launch(kernel):
     """HBM bytes per launch of a profiled GEMM kernel from its committed counter
     summary, or None."""
@@ -246,6 +281,8 @@ def main():
         return copy_calibration()
     if len(sys.argv) > 1 and sys.argv[1] == "--summarize-pmc":
         return summarize_pmc(*sys.argv[2:6], " ".join(sys.argv[6:]))
+    if len(sys.argv) > 1 and sys.argv[1] == "--summarize-stats":
+        return summarize_stats(sys.argv[2], sys.argv[3], sys.argv[4], " ".join(sys.argv[5:]))
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
@@ -255,6 +292,11 @@ def main():
     ap.add_argument("--ir-method", default="fgmres", choices=["ir", "fgmres"],
                     help="FP64 refinement around the FP32 LU: classic sweeps or FGMRES")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison path")
+    ap.add_argument("--fp64-steps", type=int, default=2,
+                    help="timed FP64 iterations (after one untimed FP64 iteration on the same iterate)")
+    ap.add_argument("--ir-tol", type=float, default=IR_TOL,
+                    help="refinement stop: normwise backward error |r| / (|Z|_F |x| + |b|)")
+    ap.add_argument("--no-matvec", action="store_true", help="skip the matrix-free product measurement")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -283,8 +325,8 @@ def main():
     log(f"workload {args.workload}: F={mesh.nf} E={mesh.ne} N={N} rule={rule} params/face={kind}")
     rng = np.random.default_rng(20260101)
     params = torch.from_numpy(rng.uniform(-2, 2, size=(mesh.nf, kind)) / k0).cuda()
-    problem = ad.GstcProblem(mesh, o, waves, kind, mode="mixed", ir_tol=1e-12, ir_maxit=40, variant=args.variant,
-                             ir_method=args.ir_method)
+    problem = ad.GstcProblem(mesh, o, waves, kind, mode="mixed", ir_tol=args.ir_tol, ir_maxit=40,
+                             variant=args.variant, ir_method=args.ir_method, ir_stop="backward")
     lib = _lib.lib
     # load every kernel once on a tiny problem (code objects, workspaces)
     tiny = sm.Mesh.icosphere(1.0, 1)
@@ -295,10 +337,15 @@ def main():
     del tst, tp
     torch.cuda.synchronize()
 
-    def step(p):
-        st = problem.solve(p)
-        J, F = problem.objective(st)
-        g = problem.gradient(st, F)
+    ir_log = []  # per iterate: forward / adjoint refinement steps and end state
+
+    def step(p, prob=problem):
+        st = prob.solve(p)
+        J, F = prob.objective(st)
+        g = prob.gradient(st, F)
+        if prob is problem:
+            ir_log.append((st.ir_iterations, st.adj_iterations, st.ir_backward_error, st.adj_backward_error,
+                           st.ir_residual, st.adj_residual))
         # gradient ascent on the cone power, parameter step bounded by 1e-3/k0
         p = p + (1e-3 / k0) * g / g.abs().max()
         return p, st, J, g
@@ -306,6 +353,7 @@ def main():
     for _ in range(args.warmup):
         params, st, J, g = step(params)
         del st
+    ir_log.clear()
     # events inside the timed iteration only on the few-launch kernels (the
     # dominant wide GEMM and the far-pair quadrature): an event pair per launch
     # of the thousands of small panel / in-panel kernels would add host time
@@ -323,7 +371,7 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     s_per_iter = elapsed / args.steps
-    ir_res, ir_its = st.ir_residual, st.ir_iterations
+    timed_log = list(ir_log)
     prof = {name: prof_read(lib, PROF[name]) for name in TIMED_PROF}
     for name in TIMED_PROF:
         lib.sie_profile_enable(PROF[name], 0)
@@ -334,13 +382,25 @@ def main():
         lib.sie_profile_enable(PROF[name], 1)
     del st
     st = problem.solve(p_last)  # the timed iterate again: the same solution, now with events
+    J_re, F_re = problem.objective(st)
+    problem.gradient(st, F_re)
     torch.cuda.synchronize()
     for name in UNTIMED_PROF:
         prof[name] = prof_read(lib, PROF[name])
         lib.sie_profile_enable(PROF[name], 0)
     problem.timings = timings_keep
+    ir_log.clear()
+    fw = [e[0] for e in timed_log]
+    aw = [e[1] for e in timed_log]
+    ir_stats = {"stop": "normwise backward error |r|/(|Z|_F |x|+|b|)", "tol": args.ir_tol,
+                "forward_steps": fw, "adjoint_steps": aw,
+                "forward_steps_mean": sum(fw) / len(fw), "forward_steps_max": max(fw),
+                "adjoint_steps_mean": sum(aw) / len(aw), "adjoint_steps_max": max(aw),
+                "backward_error_max": max(max(e[2], e[3]) for e in timed_log),
+                "rel_residual_max": max(max(e[4], e[5]) for e in timed_log)}
     log(f"mixed: {s_per_iter:.3f} s/iter  phases {json.dumps({k: round(v, 3) for k, v in phases.items()})}")
-    log(f"{args.ir_method}: rel residual {ir_res:.2e} in {ir_its} steps; J={float(J):.6e}")
+    log(f"{args.ir_method}: steps per iterate forward {fw} adjoint {aw}; max backward error "
+        f"{ir_stats['backward_error_max']:.2e}, max rel residual {ir_stats['rel_residual_max']:.2e}; J={float(J):.6e}")
 
     # re-run the last iterate's solve in FP64 for the error and the speed ratio
     out_extra = {}
@@ -351,23 +411,49 @@ def main():
         del st
         torch.cuda.empty_cache()
         p64 = ad.GstcProblem(mesh, o, waves, kind, mode="fp64")
-        # timed without per-launch events (like the mixed iteration's many-launch kernels)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
+        # one untimed FP64 iteration of the last iterate (allocation, first
+        # launches; its solution is the reference for the errors), then the
+        # same iteration timed fp64_steps times like the mixed one (whole
+        # steps, no per-launch events)
         st64 = p64.solve(p_last)
         J64, F64 = p64.objective(st64)
         g64 = p64.gradient(st64, F64)
-        torch.cuda.synchronize()
-        t64 = time.perf_counter() - t1
         ex = float(torch.linalg.norm(x_mixed - st64.xs) / torch.linalg.norm(st64.xs))
         eg = float(torch.linalg.norm(g_mixed - g64) / torch.linalg.norm(g64))
         eJ = abs(J_mixed - float(J64)) / abs(float(J64))
-        log(f"fp64: {t64:.3f} s/iter  phases {json.dumps({k: round(v, 3) for k, v in p64.timings.items()})}")
-        log(f"mixed vs fp64: rel err x {ex:.2e}, grad {eg:.2e}, J {eJ:.2e}")
-        out_extra = {"fp64_s_per_iter": t64, "speedup_vs_fp64": t64 / s_per_iter, "rel_err_x_vs_fp64": ex,
-                     "rel_err_grad_vs_fp64": eg, "rel_err_J_vs_fp64": eJ,
-                     "fp64_phases_s": {k: round(v, 4) for k, v in p64.timings.items()}}
         del st64
+        p64.timings = {}
+        t64 = None
+        if args.fp64_steps > 0:
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            for _ in range(args.fp64_steps):
+                _, st64, _, _ = step(p_last, p64)
+                del st64
+            torch.cuda.synchronize()
+            t64 = (time.perf_counter() - t1) / args.fp64_steps
+            log(f"fp64: {t64:.3f} s/iter over {args.fp64_steps} iterations  phases "
+                f"{json.dumps({k: round(v / args.fp64_steps, 3) for k, v in p64.timings.items()})}")
+        log(f"mixed vs fp64: rel err x {ex:.2e}, grad {eg:.2e}, J {eJ:.2e}")
+        out_extra = {"fp64_s_per_iter": t64, "fp64_steps": args.fp64_steps,
+                     "speedup_vs_fp64": (t64 / s_per_iter) if t64 else None, "rel_err_x_vs_fp64": ex,
+                     "rel_err_grad_vs_fp64": eg, "rel_err_J_vs_fp64": eJ,
+                     "fp64_phases_s": {k: round(v / max(args.fp64_steps, 1), 4) for k, v in p64.timings.items()}}
+        torch.cuda.empty_cache()
+
+    # one matrix-free product of the same operator (the reference's
+    # recomputing matvec: its quadrature is redone for every product)
+    if not args.no_matvec:
+        recs = ad.records(p_last, kind, o)
+        xv = torch.randn(N, dtype=torch.complex128, device="cuda")
+        sm.ops.gstc_matvec(mesh, o, recs, xv)  # warm (scratch)
+        torch.cuda.synchronize()
+        lib.sie_profile_enable(PROF["matvec_free"], 1)
+        sm.ops.gstc_matvec(mesh, o, recs, xv)
+        torch.cuda.synchronize()
+        prof["matvec_free"] = prof_read(lib, PROF["matvec_free"])
+        lib.sie_profile_enable(PROF["matvec_free"], 0)
+        del recs, xv
         torch.cuda.empty_cache()
 
     rooflines = {}
@@ -375,10 +461,17 @@ def main():
         if n == 0 or ms <= 0:
             continue
         peak, bound = PEAK[name]
+        if bound == "latency":
+            rooflines[name] = {"bound": "latency", "launches": n, "avg_ms": ms / n, "total_ms": ms}
+            if name == "panel":
+                # one grid-wide pivot exchange per column of the factorisation
+                rooflines[name]["us_per_column"] = ms * 1e3 / N
+            continue
         if bound == "hbm":
-            ach = by / (ms / n * 1e-3) / n
+            ach = by / (ms * 1e-3)
             rooflines[name] = {"bound": "hbm", "achieved": ach / 1e9, "peak": peak / 1e9, "unit": "GB/s",
-                               "frac": ach / peak, "launches": n, "avg_ms": ms / n}
+                               "frac": ach / peak, "launches": n, "avg_ms": ms / n, "total_ms": ms,
+                               "alg_bytes_per_launch": by / n}
         else:
             ach = fl / (ms * 1e-3)
             rooflines[name] = {"bound": bound, "achieved": ach / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
@@ -403,10 +496,10 @@ def main():
         "vs_baseline": None, "dtype": DTYPE[args.variant], "data": "synthetic (seeded parameters, generated mesh)",
         "config": {"workload": args.workload, "faces": mesh.nf, "edges": mesh.ne, "unknowns": N,
                    "quadrature_rule": rule, "params_per_face": kind, "rhs": 1, "lu_variant": args.variant,
-                   "objective": "far-field power, 10 deg cone about +z", "ir_tol": 1e-12,
-                   "ir_method": args.ir_method},
+                   "objective": "far-field power, 10 deg cone about +z", "ir_tol": args.ir_tol,
+                   "ir_stop": "backward error", "ir_method": args.ir_method},
         "phases_s": {k: round(v, 4) for k, v in phases.items()},
-        "ir_rel_residual": ir_res, "ir_sweeps": ir_its,
+        "refinement": ir_stats,
         "roofline": roof, "rooflines": rooflines, "cpu_baseline": cpu,
     }
     line.update(out_extra)

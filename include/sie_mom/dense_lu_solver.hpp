@@ -46,9 +46,10 @@ void solve_dense_lu(int dimension, complex_t* matrix, complex_t* rhs);
 
 // A = P L U in complex FP32 computed on the FP16 matrix cores with FP32-level
 // accuracy (each FP32 product from three FP16 products of a hi/lo split), then
-// op(A) X = B solved to a relative residual tol by FP64 iterative refinement
-// against the complex128 matrix, which must stay unchanged and resident while
-// the solver is used. Owns the FP32 factors (8 dimension^2 bytes).
+// op(A) X = B solved by FP64 iterative refinement against the complex128
+// matrix, which must stay unchanged and resident while the solver is used,
+// until the normwise backward error |b - op(A) x| / (|A|_F |x| + |b|) reaches
+// tol. Owns the FP32 factors (8 dimension^2 bytes).
 class MixedLuSolver {
 public:
     MixedLuSolver() = default;
@@ -60,11 +61,15 @@ public:
     // GMRES-IR: GMRES(20) right-preconditioned by the FP32 factors (applied in
     // FP64 arithmetic) on the FP64 matrix, per column; for more than four
     // columns with op N, blocked refinement sweeps instead. maxit bounds the
-    // steps (sweeps) per column. Returns the per-column relative residuals
-    // reached (<= tol unless maxit ran out).
+    // steps (sweeps) per column. Returns the per-column normwise backward
+    // errors reached (<= tol unless maxit ran out); relative_residuals
+    // receives |r| / |b| per column.
     std::vector<double> solve(Op op, int nrhs, const complex_t* rhs, int ldb, complex_t* x, int ldx,
-                              double tol = 1e-12, int maxit = 30, std::vector<int>* sweeps = nullptr) const;
+                              double tol = 1e-15, int maxit = 30, std::vector<int>* sweeps = nullptr,
+                              std::vector<double>* relative_residuals = nullptr) const;
     int dimension() const { return n_; }
+    // Frobenius norm of the FP64 matrix of the last factorisation.
+    double matrix_norm() const { return znorm_; }
     void set_stream(void* stream) { stream_ = stream; }
 
 private:
@@ -74,6 +79,7 @@ private:
     void* lu32_ = nullptr;
     int* pivots_ = nullptr;
     void* stream_ = nullptr;
+    double znorm_ = 0.0;
 };
 
 }  // namespace gpu

@@ -31,7 +31,8 @@ struct DielectricOptions {
     // extension: FP32 matrix-core LU + FP64 refinement (GMRES-IR for one
     // right-hand side, blocked refinement sweeps for several)
     SolvePrecision precision = SolvePrecision::fp64;
-    double refinement_tolerance = 1e-12;
+    // normwise backward error |b - Z x| / (|Z|_F |x| + |b|) the refinement reaches
+    double refinement_tolerance = 1e-15;
     int refinement_max_sweeps = 40;
 };
 

@@ -28,7 +28,8 @@ struct GstcSieOptions {
     real_t vacuum_permittivity = kCodata2022Eps0;
     real_t vacuum_permeability = kCodata2022Mu0;
     SolvePrecision precision = SolvePrecision::fp64;
-    double refinement_tolerance = 1e-12;
+    // normwise backward error |b - Z x| / (|Z|_F |x| + |b|) the refinement reaches
+    double refinement_tolerance = 1e-15;
     int refinement_max_sweeps = 40;
 };
 

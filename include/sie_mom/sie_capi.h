@@ -176,7 +176,8 @@ int sie_gemm(int precision, int m, int n, int k, double alpha, const void* A_dev
  * ipiv_dev[k] (0-based) is the row swapped with row k. Replaces
  * DenseLuSolver::factorize / cusolverDnZgetrf (dense_lu_solver.cuh:42-112);
  * returns SIE_ERR_SINGULAR with "dense matrix is singular at U diagonal k"
- * (k 1-based, also stored in *info) when a pivot column is exactly zero.
+ * (k 1-based, also stored in *info) when a pivot column is exactly zero, and
+ * SIE_ERR_DEVICE when a panel's grid-wide exchange gave up (sie_set_wait_bound).
  * Synchronises the stream. variant: as sie_gemm, for the trailing update. */
 int sie_lu_factor(int precision, int n, void* A_dev, long long lda, int* ipiv_dev, int* info,
                   int variant, void* stream);
@@ -184,7 +185,9 @@ int sie_lu_factor(int precision, int n, void* A_dev, long long lda, int* ipiv_de
 /* Solve op(A) X = B in place with the factors of sie_lu_factor; B is n x nrhs
  * (ldb). Replaces DenseLuSolver::solve / cusolverDnZgetrs
  * (dense_lu_solver.cuh:122-166); op = 2 is the adjoint solve with A^H and
- * op = 1 (A^T) is solved as A^H conj(X) = conj(B). */
+ * op = 1 (A^T) is solved as A^H conj(X) = conj(B). With nrhs <= 4 (blocked
+ * sweeps that wait on published blocks) the call synchronises the stream and
+ * returns SIE_ERR_DEVICE if such a wait gave up. */
 int sie_lu_solve(int precision, int op, int n, int nrhs, const void* LU_dev, long long lda,
                  const int* ipiv_dev, void* B_dev, long long ldb, void* stream);
 
@@ -193,34 +196,56 @@ int sie_lu_solve(int precision, int op, int n, int nrhs, const void* LU_dev, lon
 int sie_zgemv(int op, int m, int n, double alpha, const double* A_dev, long long lda,
               const double* x_dev, double beta, double* y_dev, void* stream);
 
-/* B = scale * A with precision conversion (64->32, 32->64 or 64->64). */
+/* B = scale * A with precision conversion (64->32, 32->64 or 64->64).
+ * fro_norm (may be NULL; complex double sources only): receives the Frobenius
+ * norm of A, formed in the same pass (deterministic order) -- the matrix norm
+ * of the refinement's backward-error test below; then the call synchronises
+ * the stream. */
 int sie_convert(int from_precision, int to_precision, long long rows, long long cols,
                 const void* A_dev, long long lda, void* B_dev, long long ldb, double scale,
-                void* stream);
+                double* fro_norm, void* stream);
 
 /* FP64 iterative refinement: solve op(Z) X = B with Z in complex double and
  * an FP32 LU of Z (sie_lu_factor(32, ...)) as the inner solver:
  *   x = 0, r = b;  repeat: x += |r| LU32^{-1}(r/|r|);  r = b - op(Z) x (FP64)
- * until |r|/|b| <= tol or maxit sweeps. No counterpart in the reference (its
- * solves are plain FP64 cuSOLVER); rel_residual[c] and iterations[c] report
- * per column. Synchronises the stream every sweep (host norm). */
+ * until the normwise backward error reaches tol,
+ *   |r| <= tol (znorm |x| + |b|)          (2-norms of the vectors),
+ * or maxit sweeps. znorm is a norm of Z, normally its Frobenius norm from
+ * sie_convert (the minimal perturbation dZ with (Z + dZ) x = b is of rank one,
+ * so its Frobenius and 2-norms agree); znorm = 0 turns the test into the
+ * relative residual |r| <= tol |b|. Per column: rel_residual[c] = |r|/|b|,
+ * backward_error[c] = |r| / (znorm |x| + |b|) (either may be NULL),
+ * iterations[c]. A column that ends above tol is reported, not an error: the
+ * caller compares backward_error (or rel_residual) with tol. No counterpart in
+ * the reference (its solves are plain FP64 cuSOLVER). Synchronises the stream
+ * every sweep (host norm). */
 int sie_ir_solve(int op, int n, int nrhs, const double* Z_dev, long long ldz, const float* LU_dev,
                  long long ldl, const int* ipiv_dev, const double* B_dev, long long ldb, double* X_dev,
-                 long long ldx, double tol, int maxit, double* rel_residual, int* iterations,
-                 void* stream);
+                 long long ldx, double tol, double znorm, int maxit, double* rel_residual,
+                 double* backward_error, int* iterations, void* stream);
 
 /* The same solve by GMRES(restart) right-preconditioned with the FP32 LU
  * (GMRES-IR): z_k = LU32^{-1} v_k with FP64 arithmetic on the FP32 factors,
  * w = op(Z) z_k and its CGS2 orthogonalisation in FP64, x += [z_0..z_{k-1}] y;
- * each cycle ends with the true FP64 residual. Same stopping rule as
- * sie_ir_solve; iterations[c] counts inner steps (one preconditioner solve
+ * each cycle ends with the true FP64 residual. Same stopping rule and outputs
+ * as sie_ir_solve; iterations[c] counts inner steps (one preconditioner solve
  * and one FP64 GEMV each). No counterpart in the reference. Synchronises the
  * stream every step. */
 int sie_fgmres_ir_solve(int op, int n, int nrhs, const double* Z_dev, long long ldz,
                         const float* LU_dev, long long ldl, const int* ipiv_dev,
                         const double* B_dev, long long ldb, double* X_dev, long long ldx,
-                        double tol, int maxit, int restart, double* rel_residual, int* iterations,
-                        void* stream);
+                        double tol, double znorm, int maxit, int restart, double* rel_residual,
+                        double* backward_error, int* iterations, void* stream);
+
+/* Bound of the library's device-side waits (the LU panels' grid-wide pivot
+ * exchanges and the triangular sweeps' published blocks), default 5 s: a
+ * wait that reaches it ends its kernel, and the call that synchronised
+ * returns SIE_ERR_DEVICE instead of leaving the device hung (e.g. when the
+ * panel's workgroups cannot all be resident because another process holds
+ * the GPU). force_timeout (testing only): the panel exchanges behave as if a
+ * workgroup never arrived, so the next factorisation gives up at the bound.
+ * No counterpart in the reference. */
+int sie_set_wait_bound(double seconds, int force_timeout);
 
 /* Euclidean norm of n complex doubles (deterministic two-stage reduction,
  * synchronises the stream; returns -1 on error). */
@@ -297,10 +322,13 @@ int sie_gstc_param_grad(int nf, int kind, const double* records_dev, const doubl
 /* Per-kernel timing with HIP events on the launch stream (no counterpart in
  * the reference, which times phases with std::chrono). kernel: 0 FP32 GEMM,
  * 1 FP64 GEMM, 2 far-pair assembly, 3 LU panel, 4 near-pair assembly,
- * 5 per-edge gather, 6 GEMV, 7 split-FP16 GEMM (128 x 128 tiles), 8 split-FP16
- * GEMM (256 x 128 tiles, K >= 512 and N > 1024). enable(k, 1) clears and
- * starts recording; read() synchronises on the recorded events and returns
- * the launch count, summed duration and summed algorithmic flops / bytes. */
+ * 5 per-edge gather, 6 GEMV (products of at least 2^24 entries), 7 split-FP16
+ * GEMM (128 x 128 tiles), 8 split-FP16 GEMM (256 x 128 tiles, K >= 512 and
+ * N > 1024), 9 single-RHS triangular solve (both blocked sweeps), 10 adjoint
+ * mass-block contraction, 11 matrix-free operator product (whole call).
+ * enable(k, 1) clears and starts recording; read() synchronises on the
+ * recorded events and returns the launch count, summed duration and summed
+ * algorithmic flops / bytes. */
 int sie_profile_enable(int kernel, int on);
 int sie_profile_read(int kernel, int* launches, double* total_ms, double* flops, double* bytes);
 

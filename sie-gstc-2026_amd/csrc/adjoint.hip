@@ -16,6 +16,7 @@
 // With Zs xs = bs the eta0-scaled system and J = sum_d w_d |F_d|^2 / (2 eta0),
 //   dJ/dtheta = -2 Re( lambda^H (dZs/dtheta) xs ),  Zs^H lambda = dJ/d conj(xs).
 #include "sie_internal.hpp"
+#include "prof.hpp"
 
 #include <cmath>
 #include <vector>
@@ -511,10 +512,16 @@ int sie_gstc_mass_sensitivity(sie_mesh_t mesh, int rule, int basis, float mu0, f
     hipStream_t s = (hipStream_t)stream;
     Mesh& m = mesh_of(mesh);
     QuadDev& q = mesh_quad(mesh, rule, s);
+    prof().begin(PROF_CONTRACT, s);
     mass_sensitivity_kernel<<<(m.nf + 63) / 64, 64, 0, s>>>(m.nf, m.ne, q.n, m.d_V.p, m.d_F.p, m.d_edge.p,
                                                             m.d_len.p, mesh_area(mesh), q.xy.p, q.w.p, basis, mu0,
                                                             eps0, (const cplx*)lambda_dev, (const cplx*)x_dev,
                                                             (cplx*)G_dev);
+    // algorithmic bytes: lambda and x (4E complex each) read once, the 32
+    // complex sensitivities per face written, the face geometry (vertices,
+    // indices, lengths, area) read once
+    prof().end(PROF_CONTRACT, s, 0.0,
+               2.0 * 16.0 * 4 * m.ne + 32.0 * 16.0 * m.nf + (double)m.nf * (9 * 8 + 6 * 4 + 4 * 8));
     SIE_HIP(hipGetLastError());
     SIE_HIP(hipStreamSynchronize(s));
     AGUARD_END

@@ -1133,9 +1133,12 @@ struct Assembler {
             SIE_HIP(hipGetLastError());
         }
         if (nn) {
+            prof().begin(PROF_NEAR_PAIRS, s);
             near_pairs_kernel<NK><<<(nn + 127) / 128, 128, 0, s>>>((int)nn, near.p, nf, n0, Q, mesh->d_V.p,
                                                                    mesh->d_F.p, mesh->d_len.p, area.p, quad.xy.p,
                                                                    quad.w.p, k1, k2, W.p);
+            // no flop model (transcendental-bound); bytes: the blocks it adds to
+            prof().end(PROF_NEAR_PAIRS, s, 0.0, (double)nn * NK * 18 * 16.0 * 2);
             SIE_HIP(hipGetLastError());
         }
     }
@@ -1199,6 +1202,9 @@ static void assemble_generic(sie_mesh* h, const sie_calderon_params* p, const Ep
     const int blocks = KIND == K_GSTC ? 4 : (KIND == K_PMCHWT ? 2 : 1);
     // one chunk holding every source face: the gather stores every entry of Z
     const bool single = nc_max >= m.nf;
+    // a matrix-free product is timed as a whole (flops of its far-pair
+    // quadrature over every ordered face pair; bytes: x and y)
+    if (x) prof().begin(PROF_MATVEC, s);
     if (x) {
         fill_kernel<<<(unsigned)std::min<long long>((dim + 255) / 256, 1 << 16), 256, 0, s>>>(y, dim, dim, 1);
     } else if (!single) {
@@ -1236,11 +1242,20 @@ static void assemble_generic(sie_mesh* h, const sie_calderon_params* p, const Ep
                                      1.0, (double*)y, s);
             if (rc != SIE_OK) throw Error(rc, sie_last_error());
         } else {
+            prof().begin(PROF_GATHER, s);
             gather_kernel<NK, KIND><<<grid, 256, 0, s>>>(m.nf, m.ne, n0, nc, A.elist.p, L, m.d_ef.p, m.d_el.p, A.W.p,
                                                          Mblk, ep, Z, ldz, 0, single ? 1 : 0);
+            // bytes: the chunk's face-pair blocks read once, its columns of Z written
+            prof().end(PROF_GATHER, s, 0.0,
+                       (double)m.nf * nc * NK * 18 * 16.0 + (double)dim * blocks * L * 16.0 * (single ? 1 : 2));
         }
         SIE_HIP(hipGetLastError());
         SIE_HIP(hipStreamSynchronize(s));  // list buffer reused by the next chunk
+    }
+    if (x) {
+        const double Q = A.quad.n;
+        prof().end(PROF_MATVEC, s, (double)m.nf * m.nf * Q * Q * (kFarGeomFlops + NK * kFarPairFlops),
+                   2.0 * 16.0 * dim);
     }
     if (wsz * sizeof(cplx) > A.scratch_bytes) A.W.release();  // the whole-matrix scratch is not kept
 }

@@ -192,8 +192,16 @@ int gmres_run(int n, int m, int maxnit, double tol, sie_matvec_fn matvec, void* 
 // (flexible form), so x += [z_0 .. z_{k-1}] y satisfies the FP64 Arnoldi
 // relation. Every cycle ends with the true residual b - op(Z) x. maxit bounds
 // the total number of inner steps (one preconditioner solve and one FP64 GEMV each).
+// Stopping test: |r| <= tol (znorm |x| + |b|), the normwise backward error
+// (Rigal & Gaches) with znorm a norm of Z (its Frobenius norm: the minimal
+// perturbation is of rank one, so its 2- and Frobenius norms agree); znorm = 0
+// gives the relative-residual test |r| <= tol |b|. Inside a cycle |x| is
+// estimated by beta |z_0| (the preconditioned first direction) in the first
+// cycle and by the last true |x| afterwards; each cycle's end tests the true
+// residual and |x|.
 int fgmres_ir_run(int op, int n, const double* Z, long long ldz, const float* LU, long long ldl, const int* ipiv,
-                  const double* b, double* x, double tol, int m, int maxit, double* out_rel, hipStream_t s) {
+                  const double* b, double* x, double tol, double znorm, int m, int maxit, double* out_rel,
+                  double* out_be, hipStream_t s) {
     require(op == 0 || op == 2, "op must be 0 (N) or 2 (C)");
     require(n > 0 && m > 0 && maxit > 0, "invalid FGMRES-IR dimensions");
     const double bn = sie_norm2(n, b, s);
@@ -207,9 +215,11 @@ int fgmres_ir_run(int op, int n, const double* Z, long long ldz, const float* LU
     const unsigned gb = blocks_for(n);
     SIE_HIP(hipMemsetAsync(x, 0, sizeof(double) * n2, s));
     SIE_HIP(hipMemcpyAsync(r.p, b, sizeof(double) * n2, hipMemcpyDeviceToDevice, s));
-    double beta = bn;
+    double beta = bn, xn = 0.0;
+    bool have_xn = false;
+    auto bound = [&](double xnorm) { return tol * (znorm * xnorm + bn); };
     int total = 0;
-    while (beta > tol * bn && total < maxit) {
+    while (beta > bound(xn) && total < maxit) {
         gm_copy_scaled_kernel<<<gb, 256, 0, s>>>(n, r.p, beta, V.p);
         SIE_HIP(hipGetLastError());
         std::fill(g.begin(), g.end(), zc(0));
@@ -225,6 +235,12 @@ int fgmres_ir_run(int op, int n, const double* Z, long long ldz, const float* LU
             SIE_HIP(hipMemcpyAsync(zt.p, vk, sizeof(double) * n2, hipMemcpyDeviceToDevice, s));
             lu_solve_f32_f64(op, n, LU, ldl, ipiv, zt.p, s);
             SIE_HIP(hipMemcpyAsync(zk, zt.p, sizeof(double) * n2, hipMemcpyDeviceToDevice, s));
+            if (!have_xn && znorm > 0) {
+                const double zn = sie_norm2(n, zk, s);
+                if (!std::isfinite(zn) || zn < 0) throw Error(SIE_ERR_NONFINITE, "FGMRES-IR preconditioned vector is not finite");
+                xn = beta * zn;
+                have_xn = true;
+            }
             check(sie_zgemv(op, n, n, 1.0, Z, ldz, zk, 0.0, w, s));
             check(sie_zgemv(2, n, k + 1, 1.0, V.p, n, w, 0.0, h.p, s));
             check(sie_zgemv(0, n, k + 1, -1.0, V.p, n, h.p, 1.0, w, s));
@@ -233,6 +249,7 @@ int fgmres_ir_run(int op, int n, const double* Z, long long ldz, const float* LU
             SIE_HIP(hipMemcpyAsync(hv.data(), h.p, sizeof(zc) * (k + 1), hipMemcpyDeviceToHost, s));
             SIE_HIP(hipMemcpyAsync(hv2.data(), h2.p, sizeof(zc) * (k + 1), hipMemcpyDeviceToHost, s));
             const double hn = sie_norm2(n, w, s);  // synchronises the stream
+            check_device_waits(s);
             if (!std::isfinite(hn) || hn < 0) throw Error(SIE_ERR_NONFINITE, "FGMRES-IR Arnoldi vector is not finite");
             zc* col = &H[(size_t)k * ldH];
             for (int j = 0; j <= k; ++j) col[j] = hv[j] + hv2[j];
@@ -252,7 +269,7 @@ int fgmres_ir_run(int op, int n, const double* Z, long long ldz, const float* LU
             const zc gk = g[k];
             g[k] = cs[k] * gk;
             g[k + 1] = -std::conj(sn[k]) * gk;
-            if (std::abs(g[k + 1]) <= tol * bn || hn == 0) {
+            if (std::abs(g[k + 1]) <= bound(xn) || hn == 0) {
                 ++k;
                 break;
             }
@@ -272,8 +289,13 @@ int fgmres_ir_run(int op, int n, const double* Z, long long ldz, const float* LU
         check(sie_zgemv(op, n, n, -1.0, Z, ldz, x, 1.0, r.p, s));
         beta = sie_norm2(n, r.p, s);
         if (!std::isfinite(beta) || beta < 0) throw Error(SIE_ERR_NONFINITE, "FGMRES-IR diverged (non-finite residual)");
+        if (znorm > 0) {
+            xn = sie_norm2(n, x, s);
+            have_xn = true;
+        }
     }
     if (out_rel) *out_rel = beta / bn;
+    if (out_be) *out_be = beta / (znorm * xn + bn);
     return total;
 }
 
@@ -315,16 +337,20 @@ int sie_gmres(int n, int m, int maxnit, double tol, sie_matvec_fn matvec, void* 
 
 int sie_fgmres_ir_solve(int op, int n, int nrhs, const double* Z, long long ldz, const float* LU, long long ldl,
                         const int* ipiv, const double* B, long long ldb, double* X, long long ldx, double tol,
-                        int maxit, int restart, double* rel_residual, int* iterations, void* stream) {
+                        double znorm, int maxit, int restart, double* rel_residual, double* backward_error,
+                        int* iterations, void* stream) {
     try {
         require(n > 0 && nrhs > 0 && ldz >= n && ldl >= n && ldb >= n && ldx >= n, "invalid FGMRES-IR dimensions");
         require(Z && LU && ipiv && B && X, "null FGMRES-IR argument");
         require(tol > 0 && std::isfinite(tol), "FGMRES-IR tolerance must be positive");
+        require(znorm >= 0 && std::isfinite(znorm), "FGMRES-IR matrix norm must be finite and non-negative");
         for (int c = 0; c < nrhs; ++c) {
-            double rel = 0;
+            double rel = 0, be = 0;
             const int it = fgmres_ir_run(op, n, Z, ldz, LU, ldl, ipiv, B + 2 * (long long)c * ldb,
-                                         X + 2 * (long long)c * ldx, tol, restart, maxit, &rel, (hipStream_t)stream);
+                                         X + 2 * (long long)c * ldx, tol, znorm, restart, maxit, &rel, &be,
+                                         (hipStream_t)stream);
             if (rel_residual) rel_residual[c] = rel;
+            if (backward_error) backward_error[c] = be;
             if (iterations) iterations[c] = it;
         }
         set_last_error("");

@@ -62,35 +62,39 @@ void MixedLuSolver::factorize(const complex_t* matrix, int dimension, long long 
     }
     A_ = matrix;
     ld_ = ld;
-    detail::check(sie_convert(64, 32, dimension, dimension, matrix, ld, lu32_, dimension, 1.0, stream_));
+    // the FP32 copy and the Frobenius norm of the FP64 operator in one pass
+    detail::check(sie_convert(64, 32, dimension, dimension, matrix, ld, lu32_, dimension, 1.0, &znorm_, stream_));
     int info = 0;
     detail::check(sie_lu_factor(32, dimension, lu32_, dimension, pivots_, &info, 2, stream_));
 }
 
 std::vector<double> MixedLuSolver::solve(Op op, int nrhs, const complex_t* rhs, int ldb, complex_t* x, int ldx,
-                                         double tol, int maxit, std::vector<int>* sweeps) const {
+                                         double tol, int maxit, std::vector<int>* sweeps,
+                                         std::vector<double>* relative_residuals) const {
     if (n_ <= 0 || !A_) throw std::invalid_argument("MixedLuSolver must be factorized before solve");
     if (op == Op::T) throw std::invalid_argument("MixedLuSolver supports op N and C");
-    std::vector<double> rel(nrhs);
+    std::vector<double> rel(nrhs), be(nrhs);
     std::vector<int> its(nrhs);
     if (op == Op::N && nrhs > 4) {
         // many columns: blocked refinement sweeps (one FP64 GEMM residual and one
         // multi-RHS FP32 solve per sweep)
         detail::check(sie_ir_solve((int)op, n_, nrhs, reinterpret_cast<const double*>(A_), ld_,
                                    static_cast<const float*>(lu32_), n_, pivots_, reinterpret_cast<const double*>(rhs),
-                                   ldb > 0 ? ldb : n_, reinterpret_cast<double*>(x), ldx > 0 ? ldx : n_, tol, maxit,
-                                   rel.data(), its.data(), stream_));
+                                   ldb > 0 ? ldb : n_, reinterpret_cast<double*>(x), ldx > 0 ? ldx : n_, tol, znorm_,
+                                   maxit, rel.data(), be.data(), its.data(), stream_));
         if (sweeps) *sweeps = its;
-        return rel;
+        if (relative_residuals) *relative_residuals = rel;
+        return be;
     }
     // GMRES-IR (FP32 factors applied in FP64 as the preconditioner), restart 20
     detail::check(sie_fgmres_ir_solve((int)op, n_, nrhs, reinterpret_cast<const double*>(A_), ld_,
                                       static_cast<const float*>(lu32_), n_, pivots_,
                                       reinterpret_cast<const double*>(rhs), ldb > 0 ? ldb : n_,
-                                      reinterpret_cast<double*>(x), ldx > 0 ? ldx : n_, tol, maxit, 20, rel.data(),
-                                      its.data(), stream_));
+                                      reinterpret_cast<double*>(x), ldx > 0 ? ldx : n_, tol, znorm_, maxit, 20,
+                                      rel.data(), be.data(), its.data(), stream_));
     if (sweeps) *sweeps = its;
-    return rel;
+    if (relative_residuals) *relative_residuals = rel;
+    return be;
 }
 
 }  // namespace gpu

@@ -170,11 +170,11 @@ public:
         }
         DeviceArray<complex_t> B((size_t)n_ * nrhs);
         host::hip_ok(hipMemcpy(B.data(), X.data(), sizeof(complex_t) * n_ * nrhs, hipMemcpyDeviceToDevice), "copy RHS");
-        const auto rel = mixed_.solve(gpu::Op::N, nrhs, B.data(), n_, X.data(), n_, o_.refinement_tolerance,
-                                      o_.refinement_max_sweeps);
-        for (double r : rel)
+        const auto be = mixed_.solve(gpu::Op::N, nrhs, B.data(), n_, X.data(), n_, o_.refinement_tolerance,
+                                     o_.refinement_max_sweeps);
+        for (double r : be)
             if (!(r <= o_.refinement_tolerance))
-                throw std::runtime_error("iterative refinement did not reach the requested residual (" +
+                throw std::runtime_error("iterative refinement did not reach the requested backward error (" +
                                          std::to_string(r) + ")");
     }
 
@@ -479,11 +479,11 @@ DielectricBatchResult solve_dielectric_dense_batch(const mesh::MeshData& mesh, c
         mixed.factorize(Z.data(), n);
         DeviceArray<complex_t> Bd((size_t)n * nrhs);
         Bd.upload(B.data(), B.size());
-        const auto rel = mixed.solve(gpu::Op::N, nrhs, Bd.data(), n, X.data(), n, o.refinement_tolerance,
-                                     o.refinement_max_sweeps);
-        for (double r : rel)
+        const auto be = mixed.solve(gpu::Op::N, nrhs, Bd.data(), n, X.data(), n, o.refinement_tolerance,
+                                    o.refinement_max_sweeps);
+        for (double r : be)
             if (!(r <= o.refinement_tolerance))
-                throw std::runtime_error("iterative refinement did not reach the requested residual (" +
+                throw std::runtime_error("iterative refinement did not reach the requested backward error (" +
                                          std::to_string(r) + ")");
     }
     host::hip_ok(hipDeviceSynchronize(), "PMCHWT solve");

@@ -42,6 +42,32 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 
+// Bound of the device-side waits: the panels' grid-wide exchanges and the
+// sweeps' published entries. ticks counts the 100 MHz constant clock
+// (s_memrealtime). A wait that reaches the bound writes a nonzero code into
+// *err and its kernel returns; every exchange also gives up as soon as *err
+// is set, so one stalled workgroup ends the whole grid. The host reads *err
+// at its next synchronisation and reports SIE_ERR_DEVICE. force (testing):
+// the panel exchanges wait as if a workgroup never arrived, so they give up
+// at the bound.
+struct WaitArg {
+    unsigned long long ticks;
+    unsigned int* err;
+    int force;
+};
+// codes in *err
+constexpr unsigned kWaitPanel = 1, kWaitSweep = 2, kWaitRow = 3;
+
+__device__ __forceinline__ void flag_wait_error(unsigned int* err, unsigned code) {
+    unsigned zero = 0;
+    __hip_atomic_compare_exchange_strong(err, &zero, code, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ bool wait_error_set(const unsigned int* err) {
+    return __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+}
+
 namespace {
 
 // =================================================================== GEMM
@@ -674,17 +700,29 @@ __global__ __launch_bounds__(512, 1) void cgemm_f16x3_r3_kernel(int M, int N, in
 
 // =================================================================== panel
 // Grid barrier over `nblocks` co-resident workgroups: monotone counter.
-__device__ __forceinline__ void grid_barrier(unsigned int* counter, unsigned int target) {
+// Bounded (WaitArg): returns true, uniformly over the workgroup, when the
+// barrier gave up; the caller then returns.
+__device__ __forceinline__ bool grid_barrier(unsigned int* counter, unsigned int target, const WaitArg& wa) {
+    __shared__ int s_stalled;
     __syncthreads();
     if (threadIdx.x == 0) {
         __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         // relaxed polling, then one acquire fence
-        while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+        int stalled = 0;
+        while (wa.force || __hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__builtin_amdgcn_s_memrealtime() - t0 > wa.ticks || wait_error_set(wa.err)) {
+                stalled = 1;
+                break;
+            }
             __builtin_amdgcn_s_sleep(1);
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (stalled) flag_wait_error(wa.err, kWaitPanel);
+        else __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        s_stalled = stalled;
     }
     __syncthreads();
+    return s_stalled != 0;
 }
 
 template <typename T>
@@ -716,10 +754,13 @@ __global__ __launch_bounds__(BS) void panel_getf2_kernel(int M, int w, cx<T>* __
                                                          Cand<T>* __restrict__ cand, cx<T>* __restrict__ pub,
                                                          cx<T>* __restrict__ drow, unsigned int* __restrict__ counter,
                                                          unsigned int counter_base, int* __restrict__ info,
-                                                         int info_base) {
+                                                         int info_base, WaitArg wa) {
     const int G = gridDim.x;
     const int r = blockIdx.x * BS + threadIdx.x;  // owned row
     const bool own = r < M;
+    // identity pivots until each column's choice is written (by thread 0
+    // after this workgroup's barriers): a launch that gives up leaves valid ones
+    if (blockIdx.x == 0 && threadIdx.x < w) ipiv[threadIdx.x] = threadIdx.x + ipiv_off;
     cx<T> a[W];
 #pragma unroll
     for (int k = 0; k < W; ++k) a[k] = (own && k < w) ? P[(long long)k * lda + r] : cx<T>{0, 0};
@@ -768,8 +809,8 @@ __global__ __launch_bounds__(BS) void panel_getf2_kernel(int M, int w, cx<T>* __
 #pragma unroll
             for (int k = 0; k < W; ++k) drow[(size_t)j * W + k] = a[k];
         }
-        // 2. grid barrier
-        grid_barrier(counter, counter_base + (unsigned)(G * (j + 1)));
+        // 2. grid barrier (a stalled exchange ends the launch; the host reports it)
+        if (grid_barrier(counter, counter_base + (unsigned)(G * (j + 1)), wa)) return;
         // 3. global winner (every workgroup computes the same); row j is
         // loaded in the same round trip as the candidates
         if (threadIdx.x < W) sdrow[threadIdx.x] = drow[(size_t)j * W + threadIdx.x];
@@ -897,8 +938,9 @@ __device__ __forceinline__ void publish_cx(cx<T>* p, cx<T> v) {
 // index), so the wait ends; the bound (seconds) only keeps a broken invariant
 // from hanging the device.
 template <typename T>
-__device__ __forceinline__ cx<T> wait_published(const cx<T>* p) {
+__device__ __forceinline__ cx<T> wait_published(const cx<T>* p, unsigned int* err) {
     unsigned long long w0 = kUnsolved, w1 = kUnsolved;
+    bool seen = false;
     for (unsigned it = 0; it < (1u << 22); ++it) {
         if constexpr (sizeof(T) == 8) {
             w0 = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(&p->re), __ATOMIC_RELAXED,
@@ -909,9 +951,13 @@ __device__ __forceinline__ cx<T> wait_published(const cx<T>* p) {
             w0 = w1 = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED,
                                         __HIP_MEMORY_SCOPE_AGENT);
         }
-        if (w0 != kUnsolved && w1 != kUnsolved) break;
+        if (w0 != kUnsolved && w1 != kUnsolved) {
+            seen = true;
+            break;
+        }
         __builtin_amdgcn_s_sleep(2);
     }
+    if (!seen) flag_wait_error(err, kWaitSweep);
     if constexpr (sizeof(T) == 8) return cx<T>{__builtin_bit_cast(T, w0), __builtin_bit_cast(T, w1)};
     else return __builtin_bit_cast(cx<T>, w0);
 }
@@ -938,17 +984,23 @@ __device__ __forceinline__ void publish_row_tagged(unsigned long long* __restric
 
 // One wave reads a tagged row of W <= 32 entries (2W words, one per lane)
 // into LDS, re-reading until every word carries the tag (bounded: the words
-// were stored before the key that led here was, so they arrive).
+// were stored before the key that led here was, so they arrive; a row still
+// incomplete at the bound is reported through *err).
 __device__ __forceinline__ void read_row_tagged(const unsigned long long* __restrict__ src, int W, unsigned tag,
-                                                float* __restrict__ out) {
+                                                float* __restrict__ out, unsigned int* err) {
     const int lane = threadIdx.x & 63;
     unsigned long long v = 0;
+    bool complete = false;
     for (unsigned it = 0; it < (1u << 20); ++it) {
         v = lane < 2 * W ? __hip_atomic_load(src + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
                          : (unsigned long long)tag << 32;
-        if (__all((unsigned)(v >> 32) == tag)) break;
+        if (__all((unsigned)(v >> 32) == tag)) {
+            complete = true;
+            break;
+        }
         __builtin_amdgcn_s_sleep(1);
     }
+    if (!complete && lane == 0) flag_wait_error(err, kWaitRow);
     if (lane < 2 * W) out[lane] = __uint_as_float((unsigned)v);
 }
 
@@ -969,8 +1021,9 @@ __global__ __launch_bounds__(BS) void panel_getf2_keyed_kernel(int M, int w, cx<
                                                                unsigned long long* __restrict__ pub,
                                                                unsigned long long* __restrict__ drow,
                                                                unsigned tag_base, int* __restrict__ info,
-                                                               int info_base) {
+                                                               int info_base, WaitArg wa) {
     static_assert(W % 8 == 0 && W <= 32, "columns are handled in groups of 8, a row read by one wave");
+    constexpr int kStalled = -2;  // swin code: the key exchange gave up (-1: singular column)
     constexpr int NG = W / 8;
     const int G = gridDim.x;
     // the key slots of the next launch (those of the previous one, which has
@@ -978,6 +1031,9 @@ __global__ __launch_bounds__(BS) void panel_getf2_keyed_kernel(int M, int w, cx<
     for (int t = blockIdx.x * BS + threadIdx.x; t < nkeys; t += G * BS) keys_next[t] = 0ull;
     const int r = blockIdx.x * BS + threadIdx.x;  // owned row
     const bool own = r < M;
+    // identity pivots until each column's choice is written (by thread 0
+    // after this workgroup's barriers): a launch that gives up leaves valid ones
+    if (blockIdx.x == 0 && threadIdx.x < w) ipiv[threadIdx.x] = threadIdx.x + ipiv_off;
     cx<float> a[W];
 #pragma unroll
     for (int k = 0; k < W; ++k) a[k] = (own && k < w) ? P[(long long)k * lda + r] : cx<float>{0, 0};
@@ -1028,6 +1084,8 @@ __global__ __launch_bounds__(BS) void panel_getf2_keyed_kernel(int M, int w, cx<
         if (threadIdx.x < 64) {
             unsigned long long best = 0;
             int bg = 0;
+            bool stalled = false;
+            const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
             for (;;) {
                 unsigned long long kv[kPanelKeyedMaxG / 64];
 #pragma unroll
@@ -1045,7 +1103,13 @@ __global__ __launch_bounds__(BS) void panel_getf2_keyed_kernel(int M, int w, cx<
                     ready = ready && kv[u] != 0;
                     if (kv[u] > best) { best = kv[u]; bg = lane + 64 * u; }
                 }
-                if (__all(ready)) break;
+                if (__all(ready) && !wa.force) break;
+                // the clock read is scalar and the error word is read by the
+                // whole wave: the exit is uniform
+                if (__builtin_amdgcn_s_memrealtime() - t0 > wa.ticks || __any(wait_error_set(wa.err))) {
+                    stalled = true;
+                    break;
+                }
                 __builtin_amdgcn_s_sleep(1);
             }
             for (int off = 32; off > 0; off >>= 1) {
@@ -1056,10 +1120,15 @@ __global__ __launch_bounds__(BS) void panel_getf2_keyed_kernel(int M, int w, cx<
             if (threadIdx.x == 0) {
                 const int bi = (int)(0xFFFFFFFFu - (unsigned)(best & 0xFFFFFFFFu));
                 swin = (best >> 32) != 0 ? (bi | (bg << 20)) : -1;
+                if (stalled) {
+                    flag_wait_error(wa.err, kWaitPanel);
+                    swin = kStalled;
+                }
             }
         }
         __syncthreads();
         const int code = swin;
+        if (code == kStalled) return;  // uniform: the host reports the stalled exchange
         if (code < 0) {
             // exactly singular column: record, keep going without a swap (zgetf2 semantics)
             if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1070,8 +1139,8 @@ __global__ __launch_bounds__(BS) void panel_getf2_keyed_kernel(int M, int w, cx<
         }
         const int p = code & 0xFFFFF, pg = code >> 20;
         // the pivot row (wave 0) and row j (wave 1), shared by the workgroup
-        if (wv == 0) read_row_tagged(pub + ((size_t)j * G + pg) * 2 * W, W, tag, (float*)sprow);
-        else if (wv == 1) read_row_tagged(drow + (size_t)j * 2 * W, W, tag, (float*)sdrow);
+        if (wv == 0) read_row_tagged(pub + ((size_t)j * G + pg) * 2 * W, W, tag, (float*)sprow, wa.err);
+        else if (wv == 1) read_row_tagged(drow + (size_t)j * 2 * W, W, tag, (float*)sdrow, wa.err);
         __syncthreads();
         if (blockIdx.x == 0 && threadIdx.x == 0) ipiv[j] = p + ipiv_off;
         // swap: position p takes old row j, position j takes the pivot row
@@ -1134,8 +1203,10 @@ __global__ __launch_bounds__(BS) void panel_getf2_keyed_kernel(int M, int w, cx<
 // Apply ipiv[k] (0-based, relative to row offset r0) for k in [k_begin, k_end)
 // in order to columns [c0, c0+nc) and [c1, c1+nc1) of A (the second range may
 // be empty). One lane per column keeps the sequence of swaps in order.
+// Pivots outside [k, rows) (never produced by a completed factorisation) are
+// skipped, so that incomplete factors cannot move memory outside the matrix.
 template <typename T>
-__global__ void laswp_kernel(cx<T>* __restrict__ A, long long lda, int c0, int nc, int c1, int nc1,
+__global__ void laswp_kernel(cx<T>* __restrict__ A, long long lda, int rows, int c0, int nc, int c1, int nc1,
                              const int* __restrict__ ipiv, int k_begin, int k_end, int r0, int reverse) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc + nc1) return;
@@ -1143,12 +1214,12 @@ __global__ void laswp_kernel(cx<T>* __restrict__ A, long long lda, int c0, int n
     if (!reverse) {
         for (int k = k_begin; k < k_end; ++k) {
             const int p = ipiv[k];
-            if (p != k) { const cx<T> t = col[k]; col[k] = col[p]; col[p] = t; }
+            if (p != k && p > k && p < rows) { const cx<T> t = col[k]; col[k] = col[p]; col[p] = t; }
         }
     } else {
         for (int k = k_end - 1; k >= k_begin; --k) {
             const int p = ipiv[k];
-            if (p != k) { const cx<T> t = col[k]; col[k] = col[p]; col[p] = t; }
+            if (p != k && p > k && p < rows) { const cx<T> t = col[k]; col[k] = col[p]; col[p] = t; }
         }
     }
 }
@@ -1399,6 +1470,33 @@ __global__ void convert_kernel(long long rows, long long cols, const cx<TI>* __r
         const cx<TI> v = A[c * lda + r];
         B[c * ldb + r] = {(TO)(v.re * scale), (TO)(v.im * scale)};
     }
+}
+
+// B = scale A (FP64 -> TO) with, per workgroup, the partial sum of |A|^2: the
+// Frobenius norm of the operator (the matrix norm of the refinement's
+// backward-error test) in the same pass over A. Fixed grid, fixed order:
+// deterministic.
+template <typename TO>
+__global__ __launch_bounds__(256) void convert_fro_kernel(long long rows, long long cols, const c64* __restrict__ A,
+                                                          long long lda, cx<TO>* __restrict__ B, long long ldb,
+                                                          double scale, double* __restrict__ part) {
+    __shared__ double s[256];
+    const long long tot = rows * cols;
+    double a = 0;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < tot;
+         t += (long long)gridDim.x * blockDim.x) {
+        const long long c = t / rows, r = t - c * rows;
+        const c64 v = A[c * lda + r];
+        a += v.re * v.re + v.im * v.im;
+        B[c * ldb + r] = {(TO)(v.re * scale), (TO)(v.im * scale)};
+    }
+    s[threadIdx.x] = a;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
 }
 
 // partial sums of |x|^2 per block (deterministic two-stage)
@@ -1793,7 +1891,7 @@ template <typename TF, typename T, int B>
 __global__ __launch_bounds__(kSweepThreads, sizeof(TF) == 4 ? 8 : 4) void trsv_chain_kernel(int n, int dir, int kb, int D, int nbulk, int op,
                                                                    const cx<TF>* __restrict__ LU, long long lda,
                                                                    cx<T>* __restrict__ vec, const cx<T>* __restrict__ inv,
-                                                                   cx<T>* __restrict__ out) {
+                                                                   cx<T>* __restrict__ out, unsigned int* err) {
     constexpr int P = kSweepThreads / B;
     __shared__ cx<T> sy[B];
     __shared__ cx<T> sres[B];
@@ -1824,14 +1922,14 @@ __global__ __launch_bounds__(kSweepThreads, sizeof(TF) == 4 ? 8 : 4) void trsv_c
             const int src = dir > 0 ? kb + q : kb - q;
             __syncthreads();
             const int w = min(B, n - src * B);
-            if (t < B) sy[t] = t < w ? wait_published(&out[(size_t)src * B + t]) : cx<T>{0, 0};
+            if (t < B) sy[t] = t < w ? wait_published(&out[(size_t)src * B + t], err) : cx<T>{0, 0};
             __syncthreads();
             sweep_tile<TF, T, B>(op, w, src * B, LU, lda, rb, n, sy, sres, spr, spi);
         }
         cx<T> m[B * B / kSweepThreads];
         trsv_block_load<T, B>(inv + (size_t)rbk * B * B, m);
         __syncthreads();
-        if (t < B) sy[t] = t < wl ? wait_published(&out[(size_t)last * B + t]) : cx<T>{0, 0};
+        if (t < B) sy[t] = t < wl ? wait_published(&out[(size_t)last * B + t], err) : cx<T>{0, 0};
         __syncthreads();
         sweep_tile_apply<TF, T, B, kSweepThreads>(op, rb, n, &stile[0][t], sy, sres, spr, spi);
         __syncthreads();
@@ -1852,7 +1950,7 @@ __global__ __launch_bounds__(kSweepThreads, sizeof(TF) == 4 ? 8 : 4) void trsv_c
         const int src = dir > 0 ? kb + q : kb - q;
         __syncthreads();
         const int w = min(B, n - src * B);
-        if (t < B) sy[t] = t < w ? wait_published(&out[(size_t)src * B + t]) : cx<T>{0, 0};
+        if (t < B) sy[t] = t < w ? wait_published(&out[(size_t)src * B + t], err) : cx<T>{0, 0};
         int o = 0;
         for (int i = wg - D; i < nbeyond; i += nbulk, ++o) {
             const int rb = (dir > 0 ? kb + 1 + D + i : i) * B;
@@ -1901,9 +1999,54 @@ struct Workspace {
         return buf.p;
     }
 };
+// The library's long-lived device state (workspaces, captured sweep graphs,
+// the wait-error word) is heap-allocated and never destroyed: at process exit
+// the HIP runtime may already be torn down, so no device call may run from a
+// static destructor. The memory goes with the process.
 static Workspace& ws(int slot) {
-    static Workspace w[8];
+    static Workspace* w = new Workspace[8];
     return w[slot];
+}
+
+// Bound of the device-side waits and its testing switch (sie_set_wait_bound).
+struct WaitCtl {
+    double seconds = 5.0;
+    int force = 0;
+};
+static WaitCtl& wait_ctl() {
+    static WaitCtl c;
+    return c;
+}
+static unsigned int* wait_word() {
+    static unsigned int* p = nullptr;
+    if (!p) {
+        unsigned int* q = nullptr;
+        SIE_HIP(hipMalloc(&q, sizeof(unsigned int)));
+        SIE_HIP(hipMemset(q, 0, sizeof(unsigned int)));
+        p = q;
+    }
+    return p;
+}
+static WaitArg wait_arg() {
+    const WaitCtl& c = wait_ctl();
+    return WaitArg{(unsigned long long)(c.seconds * 1e8), wait_word(), c.force};
+}
+// Reads (and clears) the wait-error word after the stream's work: a device
+// wait that gave up becomes SIE_ERR_DEVICE. Synchronises the stream.
+static void check_waits(hipStream_t s) {
+    unsigned int* w = wait_word();
+    unsigned int h = 0;
+    SIE_HIP(hipMemcpyAsync(&h, w, sizeof(h), hipMemcpyDeviceToHost, s));
+    SIE_HIP(hipStreamSynchronize(s));
+    if (h == 0) return;
+    SIE_HIP(hipMemsetAsync(w, 0, sizeof(unsigned int), s));
+    SIE_HIP(hipStreamSynchronize(s));
+    const char* what = h == kWaitPanel ? "a panel's grid-wide pivot exchange"
+                       : h == kWaitSweep ? "a triangular sweep's published block"
+                                         : "a panel's published pivot row";
+    throw Error(SIE_ERR_DEVICE, std::string("device wait gave up: ") + what + " did not arrive within " +
+                                    std::to_string(wait_ctl().seconds) +
+                                    " s (workgroups not co-resident, or the device is shared)");
 }
 
 template <typename T>
@@ -2025,22 +2168,22 @@ static void gemm_f16x3(int M, int N, int K, double alpha, const c32* A, long lon
 }
 
 template <typename T>
-static void laswp(cx<T>* A, long long lda, int c0, int nc, const int* ipiv, int kb, int ke, int r0, bool rev,
-                  hipStream_t s) {
+static void laswp(cx<T>* A, long long lda, int rows, int c0, int nc, const int* ipiv, int kb, int ke, int r0,
+                  bool rev, hipStream_t s) {
     if (nc <= 0 || ke <= kb) return;
-    laswp_kernel<T><<<(nc + 127) / 128, 128, 0, s>>>(A, lda, c0, nc, 0, 0, ipiv, kb, ke, r0, rev ? 1 : 0);
+    laswp_kernel<T><<<(nc + 127) / 128, 128, 0, s>>>(A, lda, rows, c0, nc, 0, 0, ipiv, kb, ke, r0, rev ? 1 : 0);
     SIE_HIP(hipGetLastError());
 }
 
 // The forward swaps of rows [kb, ke) on two column ranges in one launch (the
 // LU's columns left and right of the block just factored).
 template <typename T>
-static void laswp2(cx<T>* A, long long lda, int c0, int nc, int c1, int nc1, const int* ipiv, int kb, int ke,
-                   hipStream_t s) {
+static void laswp2(cx<T>* A, long long lda, int rows, int c0, int nc, int c1, int nc1, const int* ipiv, int kb,
+                   int ke, hipStream_t s) {
     nc = std::max(nc, 0);
     nc1 = std::max(nc1, 0);
     if (nc + nc1 <= 0 || ke <= kb) return;
-    laswp_kernel<T><<<(nc + nc1 + 127) / 128, 128, 0, s>>>(A, lda, c0, nc, c1, nc1, ipiv, kb, ke, 0, 0);
+    laswp_kernel<T><<<(nc + nc1 + 127) / 128, 128, 0, s>>>(A, lda, rows, c0, nc, c1, nc1, ipiv, kb, ke, 0, 0);
     SIE_HIP(hipGetLastError());
 }
 
@@ -2165,14 +2308,16 @@ static void panel(int M, int w, cx<T>* P, long long lda, int* ipiv, int ipiv_off
         unsigned tag_base = launch_no << 6;
         unsigned long long* pubw = (unsigned long long*)pub;
         unsigned long long* droww = (unsigned long long*)drow;
+        WaitArg wa = wait_arg();
         void* args[] = {&M, &w, &P, &lda, &ipiv, &ipiv_off, &kcur, &knext, &nkeys, &pubw, &droww, &tag_base,
-                        &d_info, &info_base};
+                        &d_info, &info_base, &wa};
         SIE_HIP(hipLaunchCooperativeKernel(kern, dim3(G), dim3(kPanelBS), args, 0, s));
     } else {
         unsigned int base = st.counter_base;
         st.counter_base += (unsigned)(G * w);  // one arrival per workgroup and column
+        WaitArg wa = wait_arg();
         void* args[] = {&M, &w, &P, &lda, &ipiv, &ipiv_off, &cand, &pub, &drow, &counter, &base, &d_info,
-                        &info_base};
+                        &info_base, &wa};
         SIE_HIP(hipLaunchCooperativeKernel(kern, dim3(G), dim3(kPanelBS), args, 0, s));
     }
     prof().end(PROF_PANEL, s, 8.0 * M * w * w / 2, 2.0 * sizeof(cx<T>) * M * w);
@@ -2255,7 +2400,7 @@ static void getrf_block(int n, cx<T>* A, long long lda, int c0, int c1, int* ipi
             const int w = std::min(W, k0 + nb - k1);
             cx<T>* P = A + (long long)k1 * lda + k1;
             panel<T>(n - k1, w, P, lda, ipiv + k1, k1, d_info, k1, s);
-            laswp2<T>(A, lda, k0, k1 - k0, k1 + w, k0 + nb - (k1 + w), ipiv, k1, k1 + w, s);
+            laswp2<T>(A, lda, n, k0, k1 - k0, k1 + w, k0 + nb - (k1 + w), ipiv, k1, k1 + w, s);
             const int rest = k0 + nb - (k1 + w);
             if (rest > 0) {
                 trsm_small<T>(w, P, lda, A + (long long)(k1 + w) * lda + k1, lda, rest, 0, s);
@@ -2266,7 +2411,7 @@ static void getrf_block(int n, cx<T>* A, long long lda, int c0, int c1, int* ipi
         }
         // this NB panel's swaps on the other columns of the block
         const int right = c1 - k0 - nb;
-        laswp2<T>(A, lda, c0, k0 - c0, k0 + nb, right, ipiv, k0, k0 + nb, s);
+        laswp2<T>(A, lda, n, c0, k0 - c0, k0 + nb, right, ipiv, k0, k0 + nb, s);
         if (right > 0) {
             trsm_lower<T>(A, lda, k0, nb, k0 + nb, right, s, variant);
             update<T>(n - k0 - nb, right, nb, A + (long long)k0 * lda + k0 + nb, A + (long long)(k0 + nb) * lda + k0,
@@ -2294,7 +2439,7 @@ static void getrf_level(int n, cx<T>* A, long long lda, int c0, int c1, int* ipi
         const int right = c1 - K0 - sp;
         // the outermost level's swaps reach the columns to its left only after
         // the factorisation (apply_deferred_swaps)
-        laswp2<T>(A, lda, c0, level == 0 ? 0 : K0 - c0, K0 + sp, right, ipiv, K0, K0 + sp, s);
+        laswp2<T>(A, lda, n, c0, level == 0 ? 0 : K0 - c0, K0 + sp, right, ipiv, K0, K0 + sp, s);
         if (right > 0) {
             trsm_lower<T>(A, lda, K0, sp, K0 + sp, right, s, variant, level + 1);
             update<T>(n - K0 - sp, right, sp, A + (long long)K0 * lda + K0 + sp, A + (long long)(K0 + sp) * lda + K0,
@@ -2338,7 +2483,7 @@ static void apply_deferred_swaps(int n, cx<T>* A, long long lda, const int* ipiv
     if (n <= bw) return;
     std::vector<int> hpiv(n);
     SIE_HIP(hipMemcpyAsync(hpiv.data(), ipiv, sizeof(int) * n, hipMemcpyDeviceToHost, s));
-    SIE_HIP(hipStreamSynchronize(s));
+    check_waits(s);  // a panel that gave up leaves ipiv incomplete: report that first
     // src rows of every block, concatenated: block L's run covers rows [a_L, n)
     std::vector<int> src;
     std::vector<size_t> off;
@@ -2435,6 +2580,7 @@ static void record_sweeps(int op, int n, const cx<TF>* LU, long long lda, cx<T>*
     SIE_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     const int per_cu = sizeof(TF) == 4 ? 2 : 1;
     const int bulk_max = std::max(1, per_cu * ncu - kSweepChain);
+    unsigned int* werr = wait_word();
     // each sweep's output starts as kUnsolved words (a kernel, not a memset
     // node, so that the sequence can be captured and replayed)
     const long long nwords = (long long)n * (long long)(sizeof(cx<T>) / 8);
@@ -2449,7 +2595,7 @@ static void record_sweeps(int op, int n, const cx<TF>* LU, long long lda, cx<T>*
             const int D = std::min(kSweepChain, nb - 1 - kb);
             const int nbulk = std::min(std::max(0, nb - (kb + 1 + D)), bulk_max);
             trsv_chain_kernel<TF, T, B><<<D + nbulk, kSweepThreads, 0, s>>>(n, 1, kb, D, nbulk, sop, LU, lda, vec,
-                                                                           inv, out);
+                                                                           inv, out, werr);
         }
     };
     // backward sweep over blocks nb-1, ..., 0
@@ -2462,7 +2608,7 @@ static void record_sweeps(int op, int n, const cx<TF>* LU, long long lda, cx<T>*
             const int D = std::min(kSweepChain, kb);
             const int nbulk = std::min(std::max(0, kb - D), bulk_max);
             trsv_chain_kernel<TF, T, B><<<D + nbulk, kSweepThreads, 0, s>>>(n, -1, kb, D, nbulk, sop, LU, lda, vec,
-                                                                           inv, out);
+                                                                           inv, out, werr);
         }
     };
     if (op == 0) {
@@ -2496,14 +2642,12 @@ struct SweepGraphCache {
     long long inv_lda = 0;
     unsigned long long inv_gen = 0;
     const void* inv_w = nullptr;
-    ~SweepGraphCache() {
-        for (auto& e : entries) (void)hipGraphExecDestroy(e.exec);
-        if (capture) (void)hipStreamDestroy(capture);
-    }
 };
+// Never destroyed (see ws()): graph executables and the capture stream are
+// released with the process.
 static SweepGraphCache& sweep_graphs() {
-    static SweepGraphCache c;
-    return c;
+    static SweepGraphCache* c = new SweepGraphCache;
+    return *c;
 }
 
 // Factors in TF, right-hand side, inverses and arithmetic in T.
@@ -2526,7 +2670,10 @@ void getrs_vec(int op, int n, const cx<TF>* LU, long long lda, const int* ipiv, 
     SIE_HIP(hipMemcpyAsync(hpiv.data(), ipiv, sizeof(int) * n, hipMemcpyDeviceToHost, s));
     SIE_HIP(hipStreamSynchronize(s));
     for (int i = 0; i < n; ++i) hp[i] = i;
-    for (int k = 0; k < n; ++k) std::swap(hp[k], hp[hpiv[k]]);
+    for (int k = 0; k < n; ++k) {
+        require(hpiv[k] >= k && hpiv[k] < n, "pivot index outside the trailing rows (factors not from sie_lu_factor?)");
+        std::swap(hp[k], hp[hpiv[k]]);
+    }
     SIE_HIP(hipMemcpyAsync(perm, hp.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
     SweepGraphCache& cache = sweep_graphs();
     // inverses of the diagonal blocks: reused while the factors are those of
@@ -2551,8 +2698,15 @@ void getrs_vec(int op, int n, const cx<TF>* LU, long long lda, const int* ipiv, 
     }
     const cx<T>* iL = op == 0 ? invL : invLh;
     const cx<T>* iU = op == 0 ? invU : invUh;
+    // timed as one unit (both sweeps); algorithmic bytes: every factor entry
+    // once (L strictly below and U on and above the diagonal), the two
+    // diagonal-block inverse sets, and the vector passes
+    const double sweep_bytes = (double)n * n * sizeof(cx<TF>) + 2.0 * nb * B * B * sizeof(cx<T>) +
+                               8.0 * n * sizeof(cx<T>);
     if (nb < 8) {  // short sequences: launch directly
+        prof().begin(PROF_SWEEPS, s);
         record_sweeps<TF, T>(op, n, LU, lda, b, iL, iU, y, t, perm, s);
+        prof().end(PROF_SWEEPS, s, 8.0 * n * n, sweep_bytes);
         return;
     }
     hipGraphExec_t exec = nullptr;
@@ -2579,7 +2733,9 @@ void getrs_vec(int op, int n, const cx<TF>* LU, long long lda, const int* ipiv, 
         SIE_HIP(hipGraphDestroy(graph));
         cache.entries.push_back(SweepGraph{tkey, op, n, LU, lda, b, w, exec});
     }
+    prof().begin(PROF_SWEEPS, s);
     SIE_HIP(hipGraphLaunch(exec, s));
+    prof().end(PROF_SWEEPS, s, 8.0 * n * n, sweep_bytes);
 }
 
 // One right-hand side in complex double through complex float LU factors
@@ -2623,7 +2779,7 @@ void getrs(int op, int n, int nrhs, const cx<T>* LU, long long lda, const int* i
         return;
     }
     if (op == 0) {
-        laswp<T>(B, ldb, 0, nrhs, ipiv, 0, n, 0, false, s);
+        laswp<T>(B, ldb, n, 0, nrhs, ipiv, 0, n, 0, false, s);
         // L y = b (unit lower), forward
         for (int k = 0; k < n; k += BS) {
             const int w = std::min(BS, n - k);
@@ -2671,7 +2827,7 @@ void getrs(int op, int n, int nrhs, const cx<T>* LU, long long lda, const int* i
             }
             trsm_small<T>(w, LU + (long long)k * lda + k, lda, B + k, ldb, nrhs, 3, s);
         }
-        laswp<T>(B, ldb, 0, nrhs, ipiv, 0, n, 0, true, s);
+        laswp<T>(B, ldb, n, 0, nrhs, ipiv, 0, n, 0, true, s);
     }
     SIE_HIP(hipGetLastError());
 }
@@ -2679,6 +2835,9 @@ void getrs(int op, int n, int nrhs, const cx<T>* LU, long long lda, const int* i
 // y = beta y + alpha op(A) x for an m x n FP64 matrix (op 0 = N, 2 = C).
 static void zgemv(int op, int m, int n, double alpha, const c64* A, long long lda, const c64* x, double beta, c64* y,
                   hipStream_t s) {
+    // timed: products of at least 2^24 entries (the operator's, not the Krylov basis')
+    const bool timed = (double)m * n >= 16777216.0;
+    if (timed) prof().begin(PROF_GEMV, s);
     if (op == 0) {
         const int chunk = 2048;
         const int nch = (n + chunk - 1) / chunk;
@@ -2692,6 +2851,8 @@ static void zgemv(int op, int m, int n, double alpha, const c64* A, long long ld
         gemv_c_partial<double, double><<<dim3((n + 3) / 4, nch), 256, 0, s>>>(m, n, chunk, A, lda, x, part);
         gemv_finish<double><<<(n + 255) / 256, 256, 0, s>>>(n, nch, part, alpha, beta, y);
     }
+    // algorithmic bytes: A once, x and y (read + write)
+    if (timed) prof().end(PROF_GEMV, s, 8.0 * m * n, 16.0 * ((double)m * n + (op == 0 ? n + 2.0 * m : m + 2.0 * n)));
     SIE_HIP(hipGetLastError());
 }
 
@@ -2700,12 +2861,12 @@ static void zgemv(int op, int m, int n, double alpha, const c64* A, long long ld
 // for all columns at once, X += |r_c| D. Stops when every column reaches tol.
 static double norm2(long long n, const c64* x, hipStream_t s);
 static void ir_blocked(int n, int nrhs, const c64* Z, long long ldz, const c32* LU, long long ldl, const int* ipiv,
-                       const c64* B, long long ldb, c64* X, long long ldx, double tol, int maxit, double* rel_out,
-                       int* it_out, hipStream_t s) {
+                       const c64* B, long long ldb, c64* X, long long ldx, double tol, double znorm, int maxit,
+                       double* rel_out, double* be_out, int* it_out, hipStream_t s) {
     DevBuf<c64> R((size_t)n * nrhs);
     DevBuf<c32> D((size_t)n * nrhs);
     DevBuf<double> nrm(nrhs), sc(nrhs), gain(nrhs);
-    std::vector<double> bn(nrhs), rn(nrhs), hs(nrhs), hg(nrhs);
+    std::vector<double> bn(nrhs), rn(nrhs), xn(nrhs, 0.0), hs(nrhs), hg(nrhs);
     const long long tot = (long long)n * nrhs;
     const unsigned g = (unsigned)std::min<long long>((tot + 255) / 256, 1 << 16);
     auto column_norms = [&](const c64* M, long long ld, std::vector<double>& out) {
@@ -2728,7 +2889,7 @@ static void ir_blocked(int n, int nrhs, const c64* Z, long long ldz, const c32* 
     for (int it = 0; it < maxit; ++it) {
         int active = 0;
         for (int c = 0; c < nrhs; ++c) {
-            if (!done[c] && rn[c] / bn[c] <= tol) done[c] = 1;
+            if (!done[c] && rn[c] <= tol * (znorm * xn[c] + bn[c])) done[c] = 1;
             // converged columns get a zero correction (their iterate is kept)
             hs[c] = done[c] ? 0.0 : 1.0 / rn[c];
             hg[c] = done[c] ? 0.0 : rn[c];
@@ -2750,12 +2911,16 @@ static void ir_blocked(int n, int nrhs, const c64* Z, long long ldz, const c32* 
         column_norms(R.p, n, rn);
         for (int c = 0; c < nrhs; ++c)
             if (!std::isfinite(rn[c])) throw Error(SIE_ERR_NONFINITE, "iterative refinement diverged (non-finite residual)");
+        if (znorm > 0) column_norms(X, ldx, xn);
     }
     for (int c = 0; c < nrhs; ++c) {
         if (rel_out) rel_out[c] = rn[c] / bn[c];
+        if (be_out) be_out[c] = rn[c] / (znorm * xn[c] + bn[c]);
         if (it_out) it_out[c] = its[c];
     }
 }
+
+void check_device_waits(hipStream_t s) { check_waits(s); }
 
 static double norm2(long long n, const c64* x, hipStream_t s) {
     const int nb = 512;
@@ -2820,7 +2985,7 @@ int sie_lu_factor(int precision, int n, void* A, long long lda, int* ipiv, int* 
     else
         getrf<double>(n, (c64*)A, lda, ipiv, d_info, s, 0);
     SIE_HIP(hipMemcpyAsync(info, d_info, sizeof(int), hipMemcpyDeviceToHost, s));
-    SIE_HIP(hipStreamSynchronize(s));
+    check_waits(s);
     if (*info > 0)
         throw Error(SIE_ERR_SINGULAR, "dense matrix is singular at U diagonal " + std::to_string(*info));
     GUARD_END
@@ -2837,6 +3002,15 @@ int sie_lu_solve(int precision, int op, int n, int nrhs, const void* LU, long lo
         getrs<float>(op, n, nrhs, (const c32*)LU, lda, ipiv, (c32*)B, ldb, s);
     else
         getrs<double>(op, n, nrhs, (const c64*)LU, lda, ipiv, (c64*)B, ldb, s);
+    if (nrhs <= 4) check_waits(s);  // the single-RHS sweeps wait on published blocks
+    GUARD_END
+}
+
+int sie_set_wait_bound(double seconds, int force_timeout) {
+    GUARD_BEGIN
+    require(seconds > 0 && seconds <= 3600, "wait bound must be in (0, 3600] seconds");
+    wait_ctl().seconds = seconds;
+    wait_ctl().force = force_timeout ? 1 : 0;
     GUARD_END
 }
 
@@ -2850,12 +3024,31 @@ int sie_zgemv(int op, int m, int n, double alpha, const double* A, long long lda
 }
 
 int sie_convert(int from_precision, int to_precision, long long rows, long long cols, const void* A, long long lda,
-                void* B, long long ldb, double scale, void* stream) {
+                void* B, long long ldb, double scale, double* fro_norm, void* stream) {
     GUARD_BEGIN
     require(rows > 0 && cols > 0 && lda >= rows && ldb >= rows, "invalid conversion dimensions");
     hipStream_t s = (hipStream_t)stream;
     const long long t = rows * cols;
     const unsigned g = (unsigned)std::min<long long>((t + 255) / 256, 1 << 16);
+    if (fro_norm) {
+        require(from_precision == 64 && (to_precision == 32 || to_precision == 64),
+                "the Frobenius norm is formed for complex double sources only");
+        double* part = (double*)ws(0).get(sizeof(double) * (1 << 16));
+        if (to_precision == 32)
+            convert_fro_kernel<float><<<g, 256, 0, s>>>(rows, cols, (const c64*)A, lda, (c32*)B, ldb, scale, part);
+        else
+            convert_fro_kernel<double><<<g, 256, 0, s>>>(rows, cols, (const c64*)A, lda, (c64*)B, ldb, scale, part);
+        SIE_HIP(hipGetLastError());
+        std::vector<double> h(g);
+        SIE_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * g, hipMemcpyDeviceToHost, s));
+        SIE_HIP(hipStreamSynchronize(s));
+        long double acc = 0;
+        for (unsigned i = 0; i < g; ++i) acc += h[i];
+        *fro_norm = std::sqrt((double)acc);
+        if (!std::isfinite(*fro_norm)) throw Error(SIE_ERR_NONFINITE, "matrix has non-finite entries");
+        set_last_error("");
+        return SIE_OK;
+    }
     if (from_precision == 64 && to_precision == 32)
         convert_kernel<double, float><<<g, 256, 0, s>>>(rows, cols, (const c64*)A, lda, (c32*)B, ldb, scale);
     else if (from_precision == 32 && to_precision == 64)
@@ -2870,17 +3063,19 @@ int sie_convert(int from_precision, int to_precision, long long rows, long long 
 
 // FP64 iterative refinement of op(Z) X = B around an FP32 LU of Z.
 int sie_ir_solve(int op, int n, int nrhs, const double* Z, long long ldz, const float* LU, long long ldl,
-                 const int* ipiv, const double* B, long long ldb, double* X, long long ldx, double tol, int maxit,
-                 double* rel_residual, int* iterations, void* stream) {
+                 const int* ipiv, const double* B, long long ldb, double* X, long long ldx, double tol, double znorm,
+                 int maxit, double* rel_residual, double* backward_error, int* iterations, void* stream) {
     GUARD_BEGIN
     require(op == 0 || op == 2, "op must be 0 (N) or 2 (C)");
     require(n > 0 && nrhs > 0 && ldz >= n && ldl >= n && ldb >= n && ldx >= n, "invalid IR dimensions");
+    require(tol > 0 && std::isfinite(tol), "refinement tolerance must be positive");
+    require(znorm >= 0 && std::isfinite(znorm), "refinement matrix norm must be finite and non-negative");
     hipStream_t s = (hipStream_t)stream;
     if (op == 0 && nrhs > 4) {
         // many right-hand sides: every sweep is one FP64 GEMM residual and one
         // multi-RHS FP32 solve on the matrix cores, columns scaled to unit norm
         ir_blocked(n, nrhs, (const c64*)Z, ldz, (const c32*)LU, ldl, ipiv, (const c64*)B, ldb, (c64*)X, ldx, tol,
-                   maxit, rel_residual, iterations, s);
+                   znorm, maxit, rel_residual, backward_error, iterations, s);
         set_last_error("");
         return SIE_OK;
     }
@@ -2894,7 +3089,7 @@ int sie_ir_solve(int op, int n, int nrhs, const double* Z, long long ldz, const 
         if (!(bn > 0) || !std::isfinite(bn)) throw Error(SIE_ERR_NONFINITE, "right-hand side norm must be finite and nonzero");
         SIE_HIP(hipMemsetAsync(x, 0, sizeof(c64) * n, s));
         SIE_HIP(hipMemcpyAsync(r.p, b, sizeof(c64) * n, hipMemcpyDeviceToDevice, s));
-        double rn = bn, rel = 1.0;
+        double rn = bn, rel = 1.0, xn = 0.0;
         int it = 0;
         for (; it < maxit; ++it) {
             // d = op(LU32)^{-1} (r / |r|), x += |r| d
@@ -2905,11 +3100,15 @@ int sie_ir_solve(int op, int n, int nrhs, const double* Z, long long ldz, const 
             zgemv(op, n, n, 1.0, (const c64*)Z, ldz, x, 0.0, r.p, s);
             residual_kernel<<<g, 256, 0, s>>>(n, b, r.p);
             rn = norm2(n, r.p, s);
+            check_waits(s);
             if (!std::isfinite(rn)) throw Error(SIE_ERR_NONFINITE, "iterative refinement diverged (non-finite residual)");
+            if (znorm > 0) xn = norm2(n, x, s);
             rel = rn / bn;
-            if (rel <= tol) { ++it; break; }
+            // normwise backward error test |r| <= tol (znorm |x| + |b|)
+            if (rn <= tol * (znorm * xn + bn)) { ++it; break; }
         }
         if (rel_residual) rel_residual[c] = rel;
+        if (backward_error) backward_error[c] = rn / (znorm * xn + bn);
         if (iterations) iterations[c] = it;
     }
     GUARD_END

@@ -10,7 +10,8 @@ namespace sie {
 
 enum ProfKernel { PROF_GEMM_F32 = 0, PROF_GEMM_F64 = 1, PROF_FAR_PAIRS = 2, PROF_PANEL = 3, PROF_NEAR_PAIRS = 4,
                   PROF_GATHER = 5, PROF_GEMV = 6, PROF_GEMM_F16X3 = 7,
-                  PROF_GEMM_F16X3_WIDE = 8, PROF_COUNT = 9 };
+                  PROF_GEMM_F16X3_WIDE = 8, PROF_SWEEPS = 9, PROF_CONTRACT = 10, PROF_MATVEC = 11,
+                  PROF_COUNT = 12 };
 
 struct ProfRec {
     hipEvent_t a, b;

@@ -30,6 +30,9 @@ inline void require(bool ok, const std::string& msg) {
 void set_last_error(const std::string& m);
 
 void lu_solve_f32_f64(int op, int n, const float* LU, long long lda, const int* ipiv, double* b, hipStream_t s);
+// Throws SIE_ERR_DEVICE when a device-side wait gave up since the last check
+// (synchronises the stream).
+void check_device_waits(hipStream_t s);
 int gmres_run(int n, int m, int maxnit, double tol, sie_matvec_fn matvec, void* user, const double* b, double* x,
               double* out_res, int* out_iters, double* history, int history_cap, int* history_len,
               hipStream_t s);

@@ -110,23 +110,51 @@ class SolveState:
     lu: DenseLuSolver
     Z: torch.Tensor = None      # FP64 operator (mixed mode)
     rec: torch.Tensor = None
+    znorm: float = 0.0          # Frobenius norm of Z (mixed mode)
     ir_residual: float = 0.0
     ir_iterations: int = 0
+    ir_backward_error: float = 0.0
+    adj_residual: float = 0.0
+    adj_iterations: int = 0
+    adj_backward_error: float = 0.0
+
+
+class RefinementNotConverged(RuntimeError):
+    """The FP64 refinement around the FP32 factors ended above its tolerance."""
 
 
 class GstcProblem:
-    """Mesh, options, excitation and cone objective of one GSTC synthesis step."""
+    """Mesh, options, excitation and cone objective of one GSTC synthesis step.
+
+    mode "fp64": FP64 LU and solves. mode "mixed": FP32-accurate LU on the
+    matrix cores and FP64 refinement of both solves to ir_tol, measured as the
+    normwise backward error |r| / (|Z|_F |x| + |b|) (ir_stop="backward") or as
+    |r| / |b| (ir_stop="residual"); a solve that ends above ir_tol raises
+    RefinementNotConverged, as the C++ drivers do."""
 
     def __init__(self, mesh, options, waves, kind, half_angle_deg=10.0, n_theta=8, n_phi=16, mode="mixed",
-                 ir_tol=1e-12, ir_maxit=30, variant="4m", ir_method="fgmres"):
+                 ir_tol=1e-15, ir_maxit=30, variant="4m", ir_method="fgmres", ir_stop="backward"):
+        if ir_stop not in ("backward", "residual"):
+            raise ValueError(f"unknown refinement stop {ir_stop!r}")
         self.mesh, self.options, self.waves, self.kind = mesh, options, waves, kind
         self.dirs, self.weights = cone_directions(half_angle_deg, n_theta, n_phi)
         self.w_dev = torch.from_numpy(self.weights).to("cuda", torch.float64)
         self.mode = mode
-        self.ir_tol, self.ir_maxit, self.ir_method = ir_tol, ir_maxit, ir_method
+        self.ir_tol, self.ir_maxit, self.ir_method, self.ir_stop = ir_tol, ir_maxit, ir_method, ir_stop
         self.variant = variant
         self.eta0 = physical_parameters(options).eta0
         self.timings = {}
+
+    def _refine(self, st, b, op):
+        znorm = st.znorm if self.ir_stop == "backward" else 0.0
+        x, rr, its, be = ir_solve(st.Z, st.lu.factors, st.lu.pivots, b, op=op, tol=self.ir_tol, maxit=self.ir_maxit,
+                                  method=self.ir_method, znorm=znorm)
+        measure = float(be[0]) if self.ir_stop == "backward" else float(rr[0])
+        if not measure <= self.ir_tol:
+            what = "backward error" if self.ir_stop == "backward" else "relative residual"
+            raise RefinementNotConverged(f"op {op} refinement ended at {what} {measure:.3e} > {self.ir_tol:.1e} "
+                                         f"after {int(its[0])} steps")
+        return x, float(rr[0]), int(its[0]), float(be[0])
 
     def _sync_time(self, key, t0):
         torch.cuda.synchronize()
@@ -150,14 +178,13 @@ class GstcProblem:
             lu.solve(xs, op="N")
             st = SolveState(xs=xs, lu=lu, rec=rec)
         else:
-            LU32 = to_precision(Z, 32)
+            LU32, znorm = to_precision(Z, 32, fro=True)
             lu = DenseLuSolver(variant=self.variant)
             lu.factorize(LU32)
             self._sync_time("factor", t0)
             t0 = time.perf_counter()
-            xs, rr, its = ir_solve(Z, LU32, lu.pivots, b, op="N", tol=self.ir_tol, maxit=self.ir_maxit,
-                                   method=self.ir_method)
-            st = SolveState(xs=xs, lu=lu, Z=Z, rec=rec, ir_residual=float(rr[0]), ir_iterations=int(its[0]))
+            st = SolveState(xs=None, lu=lu, Z=Z, rec=rec, znorm=znorm)
+            st.xs, st.ir_residual, st.ir_iterations, st.ir_backward_error = self._refine(st, b, "N")
         self._sync_time("solve", t0)
         return st
 
@@ -175,8 +202,7 @@ class GstcProblem:
             lam = g.clone()
             st.lu.solve(lam, op="C")
         else:
-            lam, _, _ = ir_solve(st.Z, st.lu.factors, st.lu.pivots, g, op="C", tol=self.ir_tol, maxit=self.ir_maxit,
-                                 method=self.ir_method)
+            lam, st.adj_residual, st.adj_iterations, st.adj_backward_error = self._refine(st, g, "C")
         self._sync_time("adjoint_solve", t0)
         t0 = time.perf_counter()
         G = mass_sensitivity(self.mesh, self.options, lam, st.xs)

@@ -94,23 +94,29 @@ def zgemv(A, x, y=None, alpha=1.0, beta=0.0, op="N"):
     return y
 
 
-def to_precision(A, precision):
-    """Column-major copy of A in complex64 (32) or complex128 (64)."""
+def to_precision(A, precision, fro=False):
+    """Column-major copy of A in complex64 (32) or complex128 (64). fro=True
+    (complex128 A) also returns the Frobenius norm of A, formed in the same
+    pass: (B, norm)."""
     rows, cols = A.shape
     dtype = torch.complex64 if precision == 32 else torch.complex128
     B = colmajor(rows, cols, dtype=dtype)
+    nrm = ctypes.c_double(0.0)
     _lib.check(_lib.lib.sie_convert(_prec(A), precision, rows, cols, _dptr(A), _ld(A), _dptr(B), _ld(B), 1.0,
-                                    _stream()))
-    return B
+                                    ctypes.byref(nrm) if fro else None, _stream()))
+    return (B, nrm.value) if fro else B
 
 
 IR_METHODS = ("ir", "fgmres")
 
 
-def ir_solve(Z, LU32, pivots, B, op="N", tol=1e-12, maxit=30, method="ir", restart=20):
+def ir_solve(Z, LU32, pivots, B, op="N", tol=1e-12, maxit=30, method="ir", restart=20, znorm=0.0):
     """FP64-accurate solve with an FP32 LU: classic iterative refinement
     (method "ir") or flexible GMRES(restart) preconditioned by the FP32 LU
-    (method "fgmres"). Returns (X, rel_residuals, iterations)."""
+    (method "fgmres"). Stops when |r| <= tol (znorm |x| + |b|): the normwise
+    backward error with znorm a norm of Z (its Frobenius norm from
+    to_precision(..., fro=True)); znorm = 0 gives |r| <= tol |b|.
+    Returns (X, rel_residuals, iterations, backward_errors)."""
     if method not in IR_METHODS:
         raise ValueError(f"unknown refinement method {method!r}")
     n = Z.shape[0]
@@ -118,15 +124,24 @@ def ir_solve(Z, LU32, pivots, B, op="N", tol=1e-12, maxit=30, method="ir", resta
     nrhs = Bm.shape[1]
     X = colmajor(n, nrhs)
     rr = np.zeros(nrhs)
+    be = np.zeros(nrhs)
     its = np.zeros(nrhs, np.int32)
     head = (OPS[op], n, nrhs, _dptr(Z), _ld(Z), _dptr(LU32), _ld(LU32), _dptr(pivots),
-            _dptr(Bm), Bm.stride(1) if nrhs > 1 else n, _dptr(X), X.stride(1) if nrhs > 1 else n, float(tol), int(maxit))
-    tail = (rr.ctypes.data_as(ctypes.c_void_p), its.ctypes.data_as(ctypes.c_void_p), _stream())
+            _dptr(Bm), Bm.stride(1) if nrhs > 1 else n, _dptr(X), X.stride(1) if nrhs > 1 else n, float(tol),
+            float(znorm), int(maxit))
+    tail = (rr.ctypes.data_as(ctypes.c_void_p), be.ctypes.data_as(ctypes.c_void_p),
+            its.ctypes.data_as(ctypes.c_void_p), _stream())
     if method == "ir":
         _lib.check(_lib.lib.sie_ir_solve(*head, *tail))
     else:
         _lib.check(_lib.lib.sie_fgmres_ir_solve(*head, int(restart), *tail))
-    return (X if B.dim() == 2 else X[:, 0]), rr, its
+    return (X if B.dim() == 2 else X[:, 0]), rr, its, be
+
+
+def set_wait_bound(seconds, force_timeout=False):
+    """Bound of the library's device-side waits (LU panel exchanges, sweep
+    blocks); force_timeout makes the panel exchanges give up (testing)."""
+    _lib.check(_lib.lib.sie_set_wait_bound(float(seconds), 1 if force_timeout else 0))
 
 
 class _DeviceVector:

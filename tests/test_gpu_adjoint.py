@@ -27,7 +27,7 @@ def make_problem(sm, level=1, kind=3, rule=5, mode="fp64", k0=4.0, basis="face",
     waves = [sm.plane_wave(o, (1, 0, 0), (0, 0, 1), "exterior")]
     rng = np.random.default_rng(seed)
     params = rng.uniform(-2, 2, size=(mesh.nf, kind)) / k0
-    prob = ad.GstcProblem(mesh, o, waves, kind, mode=mode, ir_tol=1e-13)
+    prob = ad.GstcProblem(mesh, o, waves, kind, mode=mode, ir_tol=1e-13, ir_stop="residual")
     return mesh, o, waves, prob, params
 
 

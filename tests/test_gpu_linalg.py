@@ -127,7 +127,7 @@ def test_ir_solve(sm, op, variant):
     LU32 = sm.linalg.to_precision(Zd, 32)
     lu = sm.linalg.DenseLuSolver(variant=variant)
     lu.factorize(LU32)
-    X, rr, its = sm.linalg.ir_solve(Zd, LU32, lu.pivots, to_dev(b), op=op, tol=1e-13, maxit=20)
+    X, rr, its, _ = sm.linalg.ir_solve(Zd, LU32, lu.pivots, to_dev(b), op=op, tol=1e-13, maxit=20)
     M = A if op == "N" else A.conj().T
     xr = np.linalg.solve(M, b)
     assert np.all(rr < 1e-13)
@@ -151,8 +151,8 @@ def test_fgmres_ir_solve(sm, op):
     LU32 = sm.linalg.to_precision(Zd, 32)
     lu = sm.linalg.DenseLuSolver(variant="f16x3")
     lu.factorize(LU32)
-    Xi, ri, iti = sm.linalg.ir_solve(Zd, LU32, lu.pivots, to_dev(b), op=op, tol=1e-13, maxit=80)
-    Xf, rf, itf = sm.linalg.ir_solve(Zd, LU32, lu.pivots, to_dev(b), op=op, tol=1e-13, maxit=80, method="fgmres")
+    Xi, ri, iti, _ = sm.linalg.ir_solve(Zd, LU32, lu.pivots, to_dev(b), op=op, tol=1e-13, maxit=80)
+    Xf, rf, itf, _ = sm.linalg.ir_solve(Zd, LU32, lu.pivots, to_dev(b), op=op, tol=1e-13, maxit=80, method="fgmres")
     print(f"op={op}: IR {iti[0]} sweeps ({ri[0]:.1e}), FGMRES-IR {itf[0]} steps ({rf[0]:.1e})", flush=True)
     assert ri[0] < 1e-13 and rf[0] < 1e-13
     assert itf[0] < iti[0]
@@ -172,7 +172,7 @@ def test_fgmres_ir_restarts(sm):
     LU32 = sm.linalg.to_precision(Zd, 32)
     lu = sm.linalg.DenseLuSolver()
     lu.factorize(LU32)
-    X, rr, its = sm.linalg.ir_solve(Zd, LU32, lu.pivots, to_dev(b), tol=1e-14, maxit=40, method="fgmres", restart=2)
+    X, rr, its, _ = sm.linalg.ir_solve(Zd, LU32, lu.pivots, to_dev(b), tol=1e-14, maxit=40, method="fgmres", restart=2)
     assert rr[0] < 1e-14
     xr = np.linalg.solve(A, b)
     assert np.linalg.norm(X.cpu().numpy() - xr) / np.linalg.norm(xr) < 1e-13
@@ -193,13 +193,76 @@ def test_ir_solve_many_rhs(sm, variant):
     LU32 = sm.linalg.to_precision(Zd, 32)
     lu = sm.linalg.DenseLuSolver(variant=variant)
     lu.factorize(LU32)
-    X, rr, its = sm.linalg.ir_solve(Zd, LU32, lu.pivots, to_dev(b), tol=1e-13, maxit=30)
+    X, rr, its, _ = sm.linalg.ir_solve(Zd, LU32, lu.pivots, to_dev(b), tol=1e-13, maxit=30)
     assert np.all(rr < 1e-13), rr
     assert its.max() <= 12
     xr = np.linalg.solve(A, b)
     x = X.cpu().numpy()
     for c in range(nrhs):
         assert np.linalg.norm(x[:, c] - xr[:, c]) / np.linalg.norm(xr[:, c]) < 1e-12
+
+
+@pytest.mark.parametrize("method,cond", [("ir", 1e6), ("fgmres", 1e7)])
+def test_backward_error_stop(sm, method, cond):
+    """Ill-conditioned systems with a right-hand side outside the dominant
+    range, so |x| is ~cond |b| and no solve reaches |r| / |b| = 1e-12 (FP64
+    rounding alone leaves ~u |A| |x| / |b|): that test runs to maxit and
+    reports it. With the Frobenius norm from the conversion the refinement
+    stops at the normwise backward error |r| / (|A|_F |x| + |b|) <= 1e-15,
+    which FP64 LU itself attains, and the result matches the host FP64 solve
+    to the conditioning bound. (A host simulation with FP32 LU factors takes
+    5-7 sweeps / 20 FGMRES steps; beyond cond ~1e8 the FP32 factors no
+    longer precondition.)"""
+    rng = np.random.default_rng(13)
+    n = 500
+    U, _ = np.linalg.qr(crandn(rng, n, n))
+    V, _ = np.linalg.qr(crandn(rng, n, n))
+    A = (U * np.logspace(0, -np.log10(cond), n)) @ V.conj().T
+    b = crandn(rng, n, 1)
+    Zd = to_dev(A)
+    LU32, fro = sm.linalg.to_precision(Zd, 32, fro=True)
+    assert abs(fro - np.linalg.norm(A)) <= 1e-13 * np.linalg.norm(A)
+    lu = sm.linalg.DenseLuSolver(variant="f16x3")
+    lu.factorize(LU32)
+    _, rr0, it0, _ = sm.linalg.ir_solve(Zd, LU32, lu.pivots, to_dev(b), tol=1e-12, maxit=40, method=method)
+    assert rr0[0] > 1e-12 and it0[0] == 40
+    X, rr, its, be = sm.linalg.ir_solve(Zd, LU32, lu.pivots, to_dev(b), tol=1e-15, maxit=40, method=method,
+                                        znorm=fro)
+    x = X.cpu().numpy()
+    print(f"{method}: backward error {be[0]:.2e} (rel. residual {rr[0]:.2e}) in {its[0]} steps", flush=True)
+    assert be[0] <= 1e-15 and its[0] < 40
+    host_be = np.linalg.norm(A @ x - b) / (np.linalg.norm(A) * np.linalg.norm(x) + np.linalg.norm(b))
+    assert host_be <= 2e-15
+    xr = np.linalg.solve(A, b)
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < cond * 1e-14
+
+
+def test_wait_bound_forced_timeout(sm):
+    """The panels' grid-wide exchanges are bounded: forced to give up (testing
+    switch), a factorisation returns a device error instead of hanging, every
+    kernel exits, and the next factorisation (switch off) is correct."""
+    rng = np.random.default_rng(14)
+    n = 1500  # three panel workgroups: a real grid-wide exchange
+    A = crandn(rng, n, n)
+    for prec, dt in ((32, torch.complex64), (64, torch.complex128)):
+        lu = sm.linalg.DenseLuSolver(variant="f16x3" if prec == 32 else "4m")
+        sm.linalg.set_wait_bound(0.01, force_timeout=True)
+        try:
+            with pytest.raises(sm.SieError, match="device wait gave up") as e:
+                lu.factorize(to_dev(A, dt))
+            assert e.value.code == 5
+        finally:
+            sm.linalg.set_wait_bound(5.0, force_timeout=False)
+        Ad = to_dev(A, dt)
+        lu.factorize(Ad)
+        b = crandn(rng, n, 1)
+        Bd = to_dev(b, dt)
+        lu.solve(Bd)
+        x = Bd.cpu().numpy()
+        res = np.linalg.norm(A @ x - b) / (np.linalg.norm(A) * np.linalg.norm(x))
+        assert res < (1e-14 if prec == 64 else 5e-6), (prec, res)
+    with pytest.raises(ValueError):
+        sm.linalg.set_wait_bound(0.0)
 
 
 def test_convert_grid_stride(sm):
