@@ -319,6 +319,31 @@ void ref_rhs_gstc_planewave(const double* V, const int* F, const int* FE, int nf
     }
 }
 
+// assemble_dielectric_rhs_kernel with the T-PMCHWT scaling of assemble_rhs
+// (src/sie_mom/solvers/dielectric/dielectric_solver.cu:400-430, 515-531):
+// sign +1 for an exterior source, -1 for an interior one, and sign * eta0 on
+// the magnetic rows with eta0 = sqrt(vac_mu / vac_eps) in double.
+void ref_rhs_pmchwt_planewave(const double* V, const int* F, const int* FE, int nf, int ne, int rule,
+                              const double* E0, const double* khat, double k, double eta, int region,
+                              double vac_mu, double vac_eps, double* rhs) {
+    Quad q = quad(rule);
+    sie_mom::cuda::PlaneWave pw{v3(E0), v3(khat), k, eta};
+    const real_t eta0 = std::sqrt(static_cast<real_t>(vac_mu) / static_cast<real_t>(vac_eps));
+    const real_t sign = region == 2 ? real_t(1) : real_t(-1);
+    for (int m = 0; m < nf; ++m) {
+        vec3_t tri[3]; int idx[3];
+        load_face(V, F, m, tri, idx);
+        complex__float ve[3], vh[3];
+        calculateExcitationQuadrature(tri, idx, q.pts.data(), q.w.data(), q.n, pw, ve, vh);
+        for (int i = 0; i < 3; ++i) {
+            size_t e = FE[3 * m + i];
+            complex__float a = complexScale(ve[i], sign), h = complexScale(vh[i], sign * eta0);
+            rhs[2 * e] += a.x; rhs[2 * e + 1] += a.y;
+            rhs[2 * (e + ne)] += h.x; rhs[2 * (e + ne) + 1] += h.y;
+        }
+    }
+}
+
 // assemble_pec_efie_operator_kernel (k2 passed twice, as pec_solver.cu:352-359 does).
 void ref_assemble_pec(const double* V, const int* F, const int* FE, int nf, int ne, int rule,
                       float k2, float mu_r2, float mu0, float omega, float thr, double* Z) {

@@ -621,13 +621,17 @@ void oracle_pair_blocks_all(const double* V, const int* F, int nf, int rule, flo
 // scaled = 0: the unscaled operator of assemble_Z_calderon_kernel (:24-148:
 // jw mu TL, TK, -TK, jw eps TL) and assemble_Z_mass_kernel (:282-452: no eta0
 // factors), i.e. the same expressions with eta0 = 1.
-void oracle_assemble_gstc_ex(const double* V, const int* F, const int* FE, int nf, int ne, int rule,
-                             float k1, float k2, float mu_r1, float mu_r2, float eps_r1, float eps_r2,
-                             float mu0, float eps0, float omega, float thr, const double* tensors,
-                             int uniform, int basis, int with_calderon, int with_mass, int scaled, double* Z) {
+}  // extern "C"
+
+// The core runs over the test faces with test_face(m) true and hands every
+// contribution (row, column, value) to sink, serialised.
+template <typename TestFace, typename Sink>
+static void gstc_core(const double* V, const int* F, const int* FE, int nf, int ne, int rule, float k1, float k2,
+                      float mu_r1, float mu_r2, float eps_r1, float eps_r2, float mu0, float eps0, float omega,
+                      float thr, const double* tensors, int uniform, int basis, int with_calderon, int with_mass,
+                      int scaled, TestFace test_face, Sink sink) {
     Rule q = rule_of(rule);
-    const size_t E = ne, dim = 4 * E;
-    std::memset(Z, 0, sizeof(double) * 2 * dim * dim);
+    const size_t E = ne;
     // make_eta0_scaled_calderon_factors (sie_scaling.cuh:48-61)
     const double eta0 = scaled ? std::sqrt((double)(mu0 / eps0)) : 1.0;
     const double mu1 = (double)(mu_r1 * mu0), mu2 = (double)(mu_r2 * mu0);
@@ -637,6 +641,7 @@ void oracle_assemble_gstc_ex(const double* V, const int* F, const int* FE, int n
     if (with_calderon) {
 #pragma omp parallel for schedule(dynamic, 1)
         for (int m = 0; m < nf; ++m) {
+            if (!test_face(m)) continue;
             V3 tt[3]; int ti[3];
             load(V, F, m, tt, ti);
             for (int n = 0; n < nf; ++n) {
@@ -657,14 +662,14 @@ void oracle_assemble_gstc_ex(const double* V, const int* F, const int* FE, int n
                         // serialise the scatter since edges are shared by faces.
 #pragma omp critical
                         {
-                            zadd(Z, dim, te, se, a_jj);
-                            zadd(Z, dim, te, se + E, a_jm);
-                            zadd(Z, dim, te + E, se, {-a_jm.re, -a_jm.im});
-                            zadd(Z, dim, te + E, se + E, a_mm);
-                            zadd(Z, dim, te + 2 * E, se + 2 * E, b_jj);
-                            zadd(Z, dim, te + 2 * E, se + 3 * E, b_jm);
-                            zadd(Z, dim, te + 3 * E, se + 2 * E, {-b_jm.re, -b_jm.im});
-                            zadd(Z, dim, te + 3 * E, se + 3 * E, b_mm);
+                            sink(te, se, a_jj);
+                            sink(te, se + E, a_jm);
+                            sink(te + E, se, C{-a_jm.re, -a_jm.im});
+                            sink(te + E, se + E, a_mm);
+                            sink(te + 2 * E, se + 2 * E, b_jj);
+                            sink(te + 2 * E, se + 3 * E, b_jm);
+                            sink(te + 3 * E, se + 2 * E, C{-b_jm.re, -b_jm.im});
+                            sink(te + 3 * E, se + 3 * E, b_mm);
                         }
                     }
                 }
@@ -675,6 +680,7 @@ void oracle_assemble_gstc_ex(const double* V, const int* F, const int* FE, int n
     const float e0f = scaled ? std::sqrt(mu0 / eps0) : 1.f;
     const float e0sq = e0f * e0f;
     for (int m = 0; m < nf; ++m) {
+        if (!test_face(m)) continue;
         V3 t[3]; int idx[3];
         load(V, F, m, t, idx);
         V3 u, v;
@@ -698,7 +704,7 @@ void oracle_assemble_gstc_ex(const double* V, const int* F, const int* FE, int n
                 const double f = 0.5, fq = 0.125;
                 const C Zee = R[4][j][i], Zem = R[5][j][i], Zme = R[6][j][i], Zmm = R[7][j][i];
                 const C Yee = R[0][j][i], Yem = R[1][j][i], Yme = R[2][j][i], Ymm = R[3][j][i];
-                auto add = [&](size_t ro, size_t co, C val, float s) { zadd(Z, dim, te + ro, se + co, cscl(val, s)); };
+                auto add = [&](size_t ro, size_t co, C val, float s) { sink(te + ro, se + co, cscl(val, s)); };
                 add(0, 0, cadd(cscl(Zee, f), cscl(Ymm, fq)), 1.f);
                 add(0, E, csub(cscl(Zem, f), cscl(Yme, fq)), e0f);
                 add(0, 2 * E, csub(cscl(Zee, f), cscl(Ymm, fq)), 1.f);
@@ -718,6 +724,49 @@ void oracle_assemble_gstc_ex(const double* V, const int* F, const int* FE, int n
             }
         }
     }
+}
+
+extern "C" {
+
+void oracle_assemble_gstc_ex(const double* V, const int* F, const int* FE, int nf, int ne, int rule,
+                             float k1, float k2, float mu_r1, float mu_r2, float eps_r1, float eps_r2,
+                             float mu0, float eps0, float omega, float thr, const double* tensors,
+                             int uniform, int basis, int with_calderon, int with_mass, int scaled, double* Z) {
+    const size_t dim = 4 * (size_t)ne;
+    std::memset(Z, 0, sizeof(double) * 2 * dim * dim);
+    gstc_core(V, F, FE, nf, ne, rule, k1, k2, mu_r1, mu_r2, eps_r1, eps_r2, mu0, eps0, omega, thr, tensors, uniform,
+              basis, with_calderon, with_mass, scaled, [](int) { return true; },
+              [&](size_t r, size_t c, C v) { zadd(Z, dim, r, c, v); });
+}
+
+// Selected rows of the eta0-scaled GSTC matrix: out[k][col] (row-major,
+// nrows x 4E) is row rows[k]. Only the test faces of the rows' edges are
+// integrated (every source face each), so a few rows of a large operator cost
+// a few test faces' quadrature.
+void oracle_gstc_rows(const double* V, const int* F, const int* FE, int nf, int ne, int rule, float k1, float k2,
+                      float mu_r1, float mu_r2, float eps_r1, float eps_r2, float mu0, float eps0, float omega,
+                      float thr, const double* tensors, int uniform, int basis, int nrows, const int* rows,
+                      double* out) {
+    const size_t dim = 4 * (size_t)ne;
+    std::memset(out, 0, sizeof(double) * 2 * dim * (size_t)nrows);
+    std::vector<int> slot(dim, -1);
+    std::vector<char> want_edge(ne, 0);
+    for (int k = 0; k < nrows; ++k) {
+        slot[rows[k]] = k;
+        want_edge[rows[k] % ne] = 1;
+    }
+    std::vector<char> face(nf, 0);
+    for (int m = 0; m < nf; ++m)
+        for (int j = 0; j < 3; ++j)
+            if (want_edge[FE[3 * m + j]]) face[m] = 1;
+    gstc_core(V, F, FE, nf, ne, rule, k1, k2, mu_r1, mu_r2, eps_r1, eps_r2, mu0, eps0, omega, thr, tensors, uniform,
+              basis, 1, 1, 1, [&](int m) { return face[m] != 0; },
+              [&](size_t r, size_t c, C v) {
+                  const int k = slot[r];
+                  if (k < 0) return;
+                  out[2 * ((size_t)k * dim + c)] += v.re;
+                  out[2 * ((size_t)k * dim + c) + 1] += v.im;
+              });
 }
 
 void oracle_assemble_gstc(const double* V, const int* F, const int* FE, int nf, int ne, int rule,

@@ -26,6 +26,10 @@ void oracle_assemble_gstc_ex(const double* V, const int* F, const int* FE, int n
                              float k1, float k2, float mu_r1, float mu_r2, float eps_r1, float eps_r2,
                              float mu0, float eps0, float omega, float thr, const double* tensors,
                              int uniform, int basis, int with_calderon, int with_mass, int scaled, double* Z);
+void oracle_gstc_rows(const double* V, const int* F, const int* FE, int nf, int ne, int rule, float k1, float k2,
+                      float mu_r1, float mu_r2, float eps_r1, float eps_r2, float mu0, float eps0, float omega,
+                      float thr, const double* tensors, int uniform, int basis, int nrows, const int* rows,
+                      double* out);
 void oracle_fields(const double* V, const int* F, const int* FE, int nf, int rule, const double* J,
                    const double* M, const double* obs, int nobs, double omega, double eps, double mu, double k,
                    double* E, double* H);

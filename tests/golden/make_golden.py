@@ -9,8 +9,9 @@ per-pair TL/TK blocks (regular and regularised) and singular parts for a
 coincident, an edge-adjacent, a vertex-adjacent and a separated pair; Wilton
 potentials at on-plane, off-plane and edge-line points; the 8-tensor mass
 batch; plane-wave excitation testing; full eta0-scaled GSTC (level 0, both
-bases), PEC T-EFIE (level 1) and T-PMCHWT (level 0) matrices; E and H of random
-currents on the level-1 sphere at far, near-surface and interior points.
+bases), PEC T-EFIE (level 1) and T-PMCHWT (level 0) matrices; the GSTC and
+T-PMCHWT (exterior and interior source) plane-wave right-hand sides; E and H of
+random currents on the level-1 sphere at far, near-surface and interior points.
 """
 import os
 import sys
@@ -123,6 +124,17 @@ def main():
     fn(ptr(V), ptr(F), ptr(FE), len(F), len(E), 7, ptr(e0), ptr(kh), kk, et, 2, f32(mu0), f32(eps0), ptr(rhs))
     out["rhs_gstc"] = rhs
     out["rhs_gstc_k_eta"] = np.array([kk, et])
+    # T-PMCHWT RHS, level 1, x-polarised +z plane wave in the exterior (region 2)
+    # and, with the interior medium's k and eta (eps_r = 4), in the interior (region 1)
+    fn = lib.ref_rhs_pmchwt_planewave
+    fn.argtypes = [P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, ctypes.c_double, ctypes.c_double,
+                   ctypes.c_int, ctypes.c_double, ctypes.c_double, P]
+    for region, (kk_, et_) in ((2, (k2d, np.sqrt(m18 / e18))), (1, (k1d, np.sqrt(m18 / e18) / 2.0))):
+        rp = np.zeros(2 * len(E), complex)
+        fn(ptr(V), ptr(F), ptr(FE), len(F), len(E), 7, ptr(e0), ptr(kh), kk_, et_, region, m18, e18, ptr(rp))
+        out[f"rhs_pmchwt_r{region}"] = rp
+        out[f"rhs_pmchwt_r{region}_k_eta"] = np.array([kk_, et_])
+    out["rhs_pmchwt_vac"] = np.array([m18, e18])
     # fields of random currents on the level-1 sphere at far, near-surface and interior points
     rng = np.random.default_rng(21)
     Jf = rng.normal(size=len(E)) + 1j * rng.normal(size=len(E))

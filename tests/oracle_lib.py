@@ -183,6 +183,23 @@ def oracle_gstc(mesh, p, tensors, uniform, basis, with_calderon=True, with_mass=
     return Z
 
 
+def oracle_gstc_rows(mesh, p, tensors, uniform, basis, rows):
+    """Rows `rows` (indices into [0, 4E)) of the oracle's eta0-scaled GSTC
+    matrix, len(rows) x 4E: only the test faces of those rows' edges are
+    integrated, against every source face."""
+    lib = ctypes.CDLL(ORACLE_SO)
+    rows = np.ascontiguousarray(rows, np.int32)
+    out = np.zeros((len(rows), 4 * mesh.ne), complex)
+    fn = lib.oracle_gstc_rows
+    fn.argtypes = [_P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int] + [ctypes.c_float] * 10 + \
+        [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P]
+    t = np.ascontiguousarray(tensors, complex)
+    fn(_ptr(np.ascontiguousarray(mesh.vertices, float)), _ptr(np.ascontiguousarray(mesh.faces, np.int32)),
+       _ptr(np.ascontiguousarray(mesh.face_edges, np.int32)), mesh.nf, mesh.ne, p.rule, *_fl(p, _PNAMES),
+       _ptr(t), int(uniform), basis, len(rows), _ptr(rows), _ptr(out))
+    return out
+
+
 def oracle_pec(mesh, p):
     lib = ctypes.CDLL(ORACLE_SO)
     dim = mesh.ne
@@ -243,6 +260,19 @@ def oracle_records(params, kind, bs, xs, ks, eta):
     """Lambda = [[jB, K], [-K^T, jX]] per face and its inverse (numpy), the
     record layout of src/sie_mom/gstc/tensor_io.py:118-120."""
     nf = len(params)
+    L = oracle_lambda(params, kind, bs, xs, ks)
+    t = np.array([np.sqrt(eta)] * 2 + [1 / np.sqrt(eta)] * 2)
+    bal = L * t[None, :, None] * t[None, None, :]
+    inv = np.linalg.inv(bal) * t[None, :, None] * t[None, None, :]
+    return np.concatenate([L.reshape(nf, 16), inv.reshape(nf, 16)], axis=1).ravel()
+
+
+def oracle_lambda(params, kind, bs, xs, ks):
+    """The fields-to-currents maps Lambda (nf x 4 x 4) of real parameters:
+    kind 3 the omega model (surface_parameters / omega_biisotropic.py:196-219),
+    kind 10 the general reciprocal lossless map in the order of the reference's
+    _reciprocal_basis (surface_parameters.py:110-127)."""
+    nf = len(params)
     L = np.zeros((nf, 4, 4), complex)
     p = np.asarray(params, float)
     if kind == 3:
@@ -258,10 +288,7 @@ def oracle_records(params, kind, bs, xs, ks, eta):
         K = ks * p[:, 6:10].reshape(nf, 2, 2)
         L[:, 0:2, 2:4] = K
         L[:, 2:4, 0:2] = -np.transpose(K, (0, 2, 1))
-    t = np.array([np.sqrt(eta)] * 2 + [1 / np.sqrt(eta)] * 2)
-    bal = L * t[None, :, None] * t[None, None, :]
-    inv = np.linalg.inv(bal) * t[None, :, None] * t[None, None, :]
-    return np.concatenate([L.reshape(nf, 16), inv.reshape(nf, 16)], axis=1).ravel()
+    return L
 
 
 def oracle_farfield_matrix(mesh, rule, k0, eta0, dirs):
@@ -292,6 +319,93 @@ def oracle_farfield_matrix(mesh, rule, k0, eta0, dirs):
             A[:, 0, 3 * ne + e] += coef * eta0 * npp
             A[:, 1, 3 * ne + e] += -coef * eta0 * nt
     return A.reshape(nd * 2, 4 * ne)
+
+
+def face_colors(mesh):
+    """Greedy colouring of the faces so that two faces sharing an RWG edge
+    differ (at most 4 colours: each face has 3 edge neighbours)."""
+    FE = np.asarray(mesh.face_edges)
+    owners = {}
+    for f, es in enumerate(FE):
+        for e in es:
+            owners.setdefault(int(e), []).append(f)
+    col = -np.ones(len(FE), int)
+    for f in range(len(FE)):
+        used = {col[g] for e in FE[f] for g in owners[int(e)] if g != f}
+        col[f] = min(c for c in range(5) if c not in used)
+    return col
+
+
+def oracle_mass_sensitivity(mesh, p, basis, lam, x):
+    """S[f, k] = lam^H M(e_{f,k}) x, the derivative of lam^H Z x with respect to
+    record entry k (0..31) of face f. The eta0-scaled mass operator is linear
+    in each face's record (the map and its inverse enter its Gram blocks
+    linearly) and face f's blocks couple only the 12 unknowns of its 3 edges,
+    so M(e_{f,k}) is read off the oracle's own mass assembly with unit records:
+    one assembly per colour and entry, on faces of that colour only (no two
+    share an edge, so their 12 x 12 blocks do not overlap)."""
+    nf, ne = mesh.nf, mesh.ne
+    FE = np.asarray(mesh.face_edges)
+    dofs = np.array([[b * ne + e for b in range(4) for e in FE[f]] for f in range(nf)])
+    col = face_colors(mesh)
+    S = np.zeros((nf, 32), complex)
+    for c in range(col.max() + 1):
+        faces = np.flatnonzero(col == c)
+        for k in range(32):
+            t = np.zeros((nf, 32), complex)
+            t[faces, k] = 1.0
+            Mk = oracle_gstc(mesh, p, t.ravel(), False, basis, with_calderon=False, with_mass=True)
+            for f in faces:
+                idx = dofs[f]
+                S[f, k] = np.conj(lam[idx]) @ Mk[np.ix_(idx, idx)] @ x[idx]
+    return S
+
+
+def oracle_param_grad(params, kind, scales, eta, S):
+    """-2 Re sum_k S[f, k] d rec_{f,k} / d theta_{f,p}: Lambda is linear in
+    the parameters (its derivative is the map of a unit parameter vector) and
+    d Lambda^-1 = -Lambda^-1 d Lambda Lambda^-1."""
+    params = np.asarray(params, float)
+    nf = len(params)
+    rec = oracle_records(params, kind, *scales, eta).reshape(nf, 2, 4, 4)
+    Li = rec[:, 1]
+    grad = np.zeros((nf, kind))
+    for q in range(kind):
+        unit = np.zeros((1, kind))
+        unit[0, q] = 1.0
+        dL = oracle_lambda(unit, kind, *scales)[0]
+        dLi = -Li @ dL @ Li
+        drec = np.concatenate([np.broadcast_to(dL.ravel(), (nf, 16)), dLi.reshape(nf, 16)], axis=1)
+        grad[:, q] = -2.0 * np.real(np.sum(S * drec, axis=1))
+    return grad
+
+
+def oracle_cone_objective(mesh, p, basis, rec, rhs, rule, k0, eta0, dirs, weights, adjoint=True):
+    """J = sum_d w_d |F_d|^2 / (2 eta0) of the scaled GSTC solution (dense
+    scipy solve of the oracle matrix) and, with adjoint, lambda = Z^-H A^H u
+    with u_d = w_d F_d / (2 eta0). Returns (J, x, F, lambda)."""
+    import scipy.linalg
+    Z = oracle_gstc(mesh, p, rec, False, basis)
+    lu = scipy.linalg.lu_factor(Z)
+    x = scipy.linalg.lu_solve(lu, rhs)
+    A = oracle_farfield_matrix(mesh, rule, k0, eta0, dirs)
+    F = (A @ x).reshape(-1, 2)
+    J = float(np.sum(weights * np.sum(np.abs(F) ** 2, axis=1)) / (2 * eta0))
+    lam = None
+    if adjoint:
+        u = (F * (weights / (2 * eta0))[:, None]).ravel()
+        lam = scipy.linalg.lu_solve(lu, A.conj().T @ u, trans=2)
+    return J, x, F, lam
+
+
+def oracle_gradient(mesh, p, basis, params, kind, scales, eta_ref, rhs, rule, k0, eta0, dirs, weights):
+    """CPU FP64 gradient of the cone objective with respect to the per-face
+    real parameters: dJ/dtheta = -2 Re lambda^H (dZ/dtheta) x. Returns
+    (J, grad (nf x kind), x, S)."""
+    rec = oracle_records(params, kind, *scales, eta_ref)
+    J, x, _, lam = oracle_cone_objective(mesh, p, basis, rec, rhs, rule, k0, eta0, dirs, weights)
+    S = oracle_mass_sensitivity(mesh, p, basis, lam, x)
+    return J, oracle_param_grad(params, kind, scales, eta_ref, S), x, S
 
 
 # ------------------------------------------------------------------ fields

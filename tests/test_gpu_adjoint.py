@@ -20,14 +20,15 @@ def sm():
     return sie_mom_amd
 
 
-def make_problem(sm, level=1, kind=3, rule=5, mode="fp64", k0=4.0, basis="face", seed=20260101):
+def make_problem(sm, level=1, kind=3, rule=5, mode="fp64", k0=4.0, basis="face", seed=20260101, ir_tol=1e-13,
+                 ir_stop="residual"):
     from sie_mom_amd import adjoint as ad
     mesh = sm.Mesh.icosphere(1.0, level)
     o = sm.GstcOptions(frequency_hz=sm.frequency_for_k0(k0), quadrature_rule=rule, basis=basis)
     waves = [sm.plane_wave(o, (1, 0, 0), (0, 0, 1), "exterior")]
     rng = np.random.default_rng(seed)
     params = rng.uniform(-2, 2, size=(mesh.nf, kind)) / k0
-    prob = ad.GstcProblem(mesh, o, waves, kind, mode=mode, ir_tol=1e-13, ir_stop="residual")
+    prob = ad.GstcProblem(mesh, o, waves, kind, mode=mode, ir_tol=ir_tol, ir_stop=ir_stop)
     return mesh, o, waves, prob, params
 
 
@@ -104,3 +105,48 @@ def test_autograd_function(sm):
     g = prob.gradient(st, F)
     assert torch.allclose(p.grad, g, rtol=1e-9, atol=0)
     assert abs(float(J) - float(J2)) <= 1e-12 * abs(float(J))
+
+
+def oracle_gradient_of(sm, mesh, o, waves, prob, params, kind):
+    """The CPU FP64 gradient (tests/oracle_lib.oracle_gradient) of the same
+    problem: oracle matrix, scipy solves, numpy far field, record
+    sensitivities from the oracle's mass assembly."""
+    from sie_mom_amd import adjoint as ad
+    from threadpoolctl import threadpool_limits
+    ph = sm.physical_parameters(o)
+    k0 = ph.omega * math.sqrt(o.vacuum_permeability * o.vacuum_permittivity)
+    eta = math.sqrt(o.vacuum_permeability / o.vacuum_permittivity)
+    rhs = ol.oracle_rhs(mesh, 0, o.quadrature_rule, waves[0], o.vacuum_permeability, o.vacuum_permittivity,
+                        o.vacuum_permeability, o.vacuum_permittivity)
+    basis = {"face": 0, "spherical": 1}[o.basis]
+    with threadpool_limits(limits=1, user_api="blas"):
+        return ol.oracle_gradient(mesh, sm.calderon_params(o), basis, params, kind, ad.susceptibility_scales(o), eta,
+                                  rhs, o.quadrature_rule, k0, ph.eta0, prob.dirs, prob.weights)
+
+
+@pytest.mark.parametrize("level,kind,rule,mode", [(1, 3, 5, "fp64"), (1, 10, 5, "fp64"), (2, 3, 5, "mixed"),
+                                                  (2, 10, 7, "fp64"), (2, 3, 17, "mixed")])
+def test_gradient_matches_oracle(sm, level, kind, rule, mode):
+    """Every gradient entry against the CPU FP64 gradient: <= 1e-9 of the
+    largest entry (FP64 and FP32-MFMA + refinement to a normwise backward
+    error of 1e-15), and the per-face record sensitivities behind it."""
+    from sie_mom_amd import adjoint as ad
+    mesh, o, waves, prob, params = make_problem(sm, level=level, kind=kind, rule=rule, mode=mode, ir_tol=1e-15,
+                                                ir_stop="backward")
+    print(f"oracle gradient level={level} kind={kind} rule={rule} mode={mode}", flush=True)
+    J, st, F = J_of(prob, params)
+    grad = prob.gradient(st, F).cpu().numpy()
+    Jo, go, xo, So = oracle_gradient_of(sm, mesh, o, waves, prob, params, kind)
+    assert abs(J - Jo) <= 1e-10 * abs(Jo)
+    assert np.linalg.norm(st.xs.cpu().numpy() - xo) <= 1e-10 * np.linalg.norm(xo)
+    err = np.abs(grad - go).max() / np.abs(go).max()
+    print(f"  max |grad - oracle| / max |oracle| = {err:.2e}", flush=True)
+    assert err <= 1e-9
+    if mode == "fp64":
+        # the contraction's per-face sensitivities of lambda^H Zs x (same lambda, x)
+        u = (F * (prob.w_dev / (2 * prob.eta0))[:, None]).contiguous()
+        g = ad.farfield_adjoint(mesh, o, u.view(-1), prob.dirs)
+        lam = g.clone()
+        st.lu.solve(lam, op="C")
+        G = ad.mass_sensitivity(mesh, o, lam, st.xs).cpu().numpy().reshape(-1, 32)
+        assert np.abs(G - So).max() <= 1e-9 * np.abs(So).max()

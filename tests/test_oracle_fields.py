@@ -65,3 +65,35 @@ def test_far_zone_matches_far_field(pec):
     Ef, _ = ol.fields(ol.ORACLE_SO, "oracle_", arrays, 5, J, np.zeros_like(J), R * dirs[:, :3], om, eps, mu, k)
     Eth = np.einsum("ij,ij->i", Ef, dirs[:, 3:6]) * R * np.exp(1j * k * R)
     assert np.abs(Eth + Fff[:, 0]).max() < 5e-3 * np.abs(Fff[:, 0]).max()
+
+
+def test_far_zone_electric_and_magnetic_currents(pec):
+    """The magnetic-current columns of the far-field matrix: random J and M
+    (scaled M, physical eta0 M) radiate in the far zone as minus the
+    Schelkunoff far field, F_theta = c (eta0 N_theta + L_phi), F_phi =
+    c (eta0 N_phi - L_theta), for both components and directions off the
+    x-z plane. A sign or duality error in the L term fails: M alone is
+    checked too, and its contribution is as large as J's."""
+    mesh, arrays, _, om, eps, mu, k, eta = pec
+    rng = np.random.default_rng(17)
+    ne = mesh.ne
+    J = rng.normal(size=ne) + 1j * rng.normal(size=ne)
+    Ms = rng.normal(size=ne) + 1j * rng.normal(size=ne)
+    R = 400.0
+    th = np.array([0.4, 1.1, 1.9, 2.7])
+    ph = np.array([0.3, 2.0, 3.5, 5.2])
+    st, ct, sp, cp = np.sin(th), np.cos(th), np.sin(ph), np.cos(ph)
+    dirs = np.stack([st * cp, st * sp, ct, ct * cp, ct * sp, -st, -sp, cp, 0 * th], axis=1)
+    A = ol.oracle_farfield_matrix(mesh, 5, k, eta, dirs)
+    for Jx, Mx in ((J, Ms), (0 * J, Ms)):
+        xs = np.zeros(4 * ne, complex)
+        xs[2 * ne:3 * ne] = Jx
+        xs[3 * ne:] = Mx
+        Fff = (A @ xs).reshape(-1, 2)
+        Ef, _ = ol.fields(ol.ORACLE_SO, "oracle_", arrays, 5, Jx, eta * Mx, R * dirs[:, :3], om, eps, mu, k)
+        g = R * np.exp(1j * k * R)
+        Eth = np.einsum("ij,ij->i", Ef, dirs[:, 3:6]) * g
+        Eph = np.einsum("ij,ij->i", Ef, dirs[:, 6:9]) * g
+        scale = np.abs(Fff).max()
+        assert np.abs(Eth + Fff[:, 0]).max() < 5e-3 * scale
+        assert np.abs(Eph + Fff[:, 1]).max() < 5e-3 * scale

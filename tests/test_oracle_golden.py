@@ -105,6 +105,51 @@ def test_rhs():
     np.testing.assert_array_equal(rhs, G["rhs_gstc"])
 
 
+@pytest.mark.parametrize("region", [2, 1])
+def test_pmchwt_rhs(region):
+    """T-PMCHWT plane-wave RHS: sign +1 (exterior source) / -1 (interior) and
+    sign * eta0 on the magnetic rows (dielectric_solver.cu:400-430, 515-531)."""
+    m1 = M(G["ico1_V"], G["ico1_F"], G["ico1_E"], G["ico1_FE"])
+    k, eta = G[f"rhs_pmchwt_r{region}_k_eta"]
+    vmu, veps = G["rhs_pmchwt_vac"]
+
+    class W:
+        E0 = (1.0, 0.0, 0.0)
+        khat = (0.0, 0.0, 1.0)
+    w = W()
+    w.k, w.eta, w.region = k, eta, region
+    rhs = ol.oracle_rhs(m1, 1, 7, w, vmu, veps, vmu, veps)
+    np.testing.assert_array_equal(rhs, G[f"rhs_pmchwt_r{region}"])
+
+
+R = np.load(os.path.join(os.path.dirname(__file__), "golden", "records_vectors.npz"))
+
+
+@pytest.mark.parametrize("kind", [10, 3])
+def test_records_match_reference(kind):
+    """Surface-map records from real parameters against the reference's own
+    parameter basis, omega model and record export (tests/golden/
+    make_records_golden.py): the maps bit for bit, the inverses (a different
+    solve) to 1e-13 of each face's largest entry; and the reference's
+    export checks, restated: reciprocity and skew-Hermitian balanced maps."""
+    bs, xs, ks = R["scales"]
+    eta = float(R["eta"])
+    got = ol.oracle_records(R[f"params{kind}"], kind, bs, xs, ks, eta).reshape(-1, 2, 16)
+    ref = R[f"records{kind}"].reshape(-1, 2, 16)
+    np.testing.assert_array_equal(got[:, 0], ref[:, 0])
+    scale = np.abs(ref[:, 1]).max(axis=1, keepdims=True)
+    assert (np.abs(got[:, 1] - ref[:, 1]) / scale).max() < 1e-13
+    L = got[:, 0].reshape(-1, 4, 4)
+    t = np.array([np.sqrt(eta)] * 2 + [1 / np.sqrt(eta)] * 2)
+    bal = L * t[None, :, None] * t[None, None, :]
+    sc = np.abs(bal).max(axis=(1, 2))
+    assert (np.abs(bal + np.conj(np.transpose(bal, (0, 2, 1)))).max(axis=(1, 2)) / sc).max() < 1e-15
+    assert (np.abs(bal[:, :2, :2] - bal[:, :2, :2].transpose(0, 2, 1)).max(axis=(1, 2)) / sc).max() < 1e-15
+    assert (np.abs(bal[:, 2:, :2] + bal[:, :2, 2:].transpose(0, 2, 1)).max(axis=(1, 2)) / sc).max() < 1e-15
+    Li = got[:, 1].reshape(-1, 4, 4)
+    assert np.abs(L @ Li - np.eye(4)).max() < 1e-8
+
+
 def test_fields():
     """E and H of random currents on the level-1 sphere (far, near-surface with
     singular extraction, interior points) against the reference's numerics."""
