@@ -237,6 +237,15 @@ int sie_fgmres_ir_solve(int op, int n, int nrhs, const double* Z_dev, long long 
                         double tol, double znorm, int maxit, int restart, double* rel_residual,
                         double* backward_error, int* iterations, void* stream);
 
+/* Releases the library's device state outside the mesh handles (workspaces,
+ * captured sweep graphs and their stream, the wait-error word, profiling
+ * events) after draining the device; later calls re-create what they need.
+ * Call it before the process exits while the HIP runtime is still up (the
+ * Python binding registers it with atexit): the library runs no device call
+ * from static destructors. No counterpart in the reference (its RAII buffers
+ * live per solve). */
+int sie_release(void);
+
 /* Bound of the library's device-side waits (the LU panels' grid-wide pivot
  * exchanges and the triangular sweeps' published blocks), default 5 s: a
  * wait that reaches it ends its kernel, and the call that synchronised

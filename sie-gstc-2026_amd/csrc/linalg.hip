@@ -2017,15 +2017,15 @@ static WaitCtl& wait_ctl() {
     static WaitCtl c;
     return c;
 }
+static unsigned int* g_wait_word = nullptr;
 static unsigned int* wait_word() {
-    static unsigned int* p = nullptr;
-    if (!p) {
+    if (!g_wait_word) {
         unsigned int* q = nullptr;
         SIE_HIP(hipMalloc(&q, sizeof(unsigned int)));
         SIE_HIP(hipMemset(q, 0, sizeof(unsigned int)));
-        p = q;
+        g_wait_word = q;
     }
-    return p;
+    return g_wait_word;
 }
 static WaitArg wait_arg() {
     const WaitCtl& c = wait_ctl();
@@ -3003,6 +3003,31 @@ int sie_lu_solve(int precision, int op, int n, int nrhs, const void* LU, long lo
     else
         getrs<double>(op, n, nrhs, (const c64*)LU, lda, ipiv, (c64*)B, ldb, s);
     if (nrhs <= 4) check_waits(s);  // the single-RHS sweeps wait on published blocks
+    GUARD_END
+}
+
+int sie_release(void) {
+    GUARD_BEGIN
+    SweepGraphCache& c = sweep_graphs();
+    bool held = g_wait_word != nullptr || c.capture != nullptr || !c.entries.empty();
+    for (int i = 0; i < 8; ++i) held = held || ws(i).buf.p != nullptr;
+    if (!held) {
+        set_last_error("");
+        return SIE_OK;
+    }
+    SIE_HIP(hipDeviceSynchronize());
+    for (auto& e : c.entries) SIE_HIP(hipGraphExecDestroy(e.exec));
+    c.entries.clear();
+    if (c.capture) SIE_HIP(hipStreamDestroy(c.capture));
+    c.capture = nullptr;
+    c.inv_LU = nullptr;
+    c.inv_w = nullptr;
+    c.inv_gen = 0;
+    for (int i = 0; i < 8; ++i) ws(i).buf.release();
+    if (g_wait_word) SIE_HIP(hipFree(g_wait_word));
+    g_wait_word = nullptr;
+    for (int k = 0; k < 64; ++k)
+        if (sie_profile_enable(k, 0) != SIE_OK) break;  // destroys recorded events
     GUARD_END
 }
 

@@ -4,6 +4,7 @@ The library is built in place by ``sie-gstc-2026_amd/build.sh`` (or
 ``__graft_entry__.build()``). There is no fallback: if the shared object is
 missing, importing this module raises.
 """
+import atexit
 import ctypes
 import os
 
@@ -97,6 +98,7 @@ _sig("sie_convert", I, [I, I, LL, LL, P, LL, P, LL, D, P, P])
 _sig("sie_ir_solve", I, [I, I, I, P, LL, P, LL, P, P, LL, P, LL, D, D, I, P, P, P, P])
 _sig("sie_fgmres_ir_solve", I, [I, I, I, P, LL, P, LL, P, P, LL, P, LL, D, D, I, I, P, P, P, P])
 _sig("sie_set_wait_bound", I, [D, I])
+_sig("sie_release", I, [])
 _sig("sie_norm2", D, [LL, P, P])
 MATVEC_FN = ctypes.CFUNCTYPE(I, P, P, P, P)
 _sig("sie_gmres", I, [I, I, I, D, MATVEC_FN, P, P, P, ctypes.POINTER(D), ctypes.POINTER(I), ctypes.POINTER(I), P, I,
@@ -110,3 +112,13 @@ _sig("sie_gstc_mass_sensitivity", I, [P, I, I, Fl, Fl, P, P, P, P])
 _sig("sie_gstc_param_grad", I, [I, I, P, P, D, D, D, P, P])
 _sig("sie_profile_enable", I, [I, I])
 _sig("sie_profile_read", I, [I, ctypes.POINTER(I), ctypes.POINTER(D), ctypes.POINTER(D), ctypes.POINTER(D)])
+
+
+@atexit.register
+def _release():
+    # device state is released while the HIP runtime is still up (atexit runs
+    # before the runtime's own teardown); nothing to do if no device work ran
+    try:
+        lib.sie_release()
+    except Exception:
+        pass

@@ -154,3 +154,30 @@ def plane_wave(o, electric_amplitude=(1.0, 0.0, 0.0), propagation_direction=(0.0
     w.eta = eta
     w.region = reg
     return w
+
+
+def fibonacci_directions(count):
+    """count unit vectors on the golden-angle (Fibonacci) spiral of the sphere:
+    z_i = 1 - (2 i + 1) / count in (-1, 1), azimuth i pi (3 - sqrt 5)."""
+    i = np.arange(count)
+    z = 1.0 - (2.0 * i + 1.0) / count
+    phi = i * math.pi * (3.0 - math.sqrt(5.0))
+    r = np.sqrt(1.0 - z * z)
+    return np.stack([r * np.cos(phi), r * np.sin(phi), z], axis=1)
+
+
+def fibonacci_plane_waves(o, count, region="exterior"):
+    """count unit plane waves incident along the Fibonacci directions d_i
+    (fibonacci_directions), polarised along theta_hat(d_i) for even i and
+    phi_hat(d_i) for odd i: the many-right-hand-side excitation of a
+    dielectric scatterer."""
+    waves = []
+    for i, d in enumerate(fibonacci_directions(count)):
+        th = math.acos(max(-1.0, min(1.0, d[2])))
+        ph = math.atan2(d[1], d[0])
+        if i % 2 == 0:
+            e = (math.cos(th) * math.cos(ph), math.cos(th) * math.sin(ph), -math.sin(th))
+        else:
+            e = (-math.sin(ph), math.cos(ph), 0.0)
+        waves.append(plane_wave(o, e, tuple(d), region))
+    return waves
