@@ -196,6 +196,20 @@ int sie_lu_solve(int precision, int op, int n, int nrhs, const void* LU_dev, lon
 int sie_zgemv(int op, int m, int n, double alpha, const double* A_dev, long long lda,
               const double* x_dev, double beta, double* y_dev, void* stream);
 
+/* sie_zgemv without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight refinement or serving loops, harnesses that forbid hipMalloc). The
+ * partial sums live in a static device array of 64 MiB per GPU (held until
+ * the process ends), so the call allocates nothing and returns
+ * SIE_ERR_INVALID when they would exceed it: op N needs
+ * 16 m ceil(n / 2048) bytes, op C 16 n ceil(m / 8192) (square operators up to
+ * n ~ 92,000 with op N). Same kernels, same operation order and results as
+ * sie_zgemv. Every call on one GPU shares that GPU's array: calls that
+ * overlap in time on one GPU (other streams or threads) race. Stream-ordered;
+ * no synchronisation. */
+int sie_zgemv_noalloc(int op, int m, int n, double alpha, const double* A_dev, long long lda,
+                      const double* x_dev, double beta, double* y_dev, void* stream);
+
 /* B = scale * A with precision conversion (64->32, 32->64 or 64->64).
  * fro_norm (may be NULL; complex double sources only): receives the Frobenius
  * norm of A, formed in the same pass (deterministic order) -- the matrix norm

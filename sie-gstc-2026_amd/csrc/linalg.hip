@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <cmath>
 #include <type_traits>
 
@@ -2832,28 +2833,63 @@ void getrs(int op, int n, int nrhs, const cx<T>* LU, long long lda, const int* i
     SIE_HIP(hipGetLastError());
 }
 
-// y = beta y + alpha op(A) x for an m x n FP64 matrix (op 0 = N, 2 = C).
+// y = beta y + alpha op(A) x for an m x n FP64 matrix (op 0 = N, 2 = C):
+// per-chunk partial sums (chunks of 2048 columns for op N, 8192 rows for op C)
+// into part, then a fixed-order reduction, so the result is deterministic.
+constexpr int kGemvChunkN = 2048, kGemvChunkC = 8192;
+static size_t zgemv_partial_bytes(int op, int m, int n) {
+    return op == 0 ? (size_t)((n + kGemvChunkN - 1) / kGemvChunkN) * m * sizeof(cx<double>)
+                   : (size_t)((m + kGemvChunkC - 1) / kGemvChunkC) * n * sizeof(cx<double>);
+}
+static void zgemv_launch(int op, int m, int n, double alpha, const c64* A, long long lda, const c64* x, double beta,
+                         c64* y, cx<double>* part, hipStream_t s) {
+    if (op == 0) {
+        const int nch = (n + kGemvChunkN - 1) / kGemvChunkN;
+        gemv_n_partial<double, double><<<dim3((m + 255) / 256, nch), 256, 0, s>>>(m, n, kGemvChunkN, A, lda, x, part);
+        gemv_finish<double><<<(m + 255) / 256, 256, 0, s>>>(m, nch, part, alpha, beta, y);
+    } else {
+        const int nch = (m + kGemvChunkC - 1) / kGemvChunkC;
+        gemv_c_partial<double, double><<<dim3((n + 3) / 4, nch), 256, 0, s>>>(m, n, kGemvChunkC, A, lda, x, part);
+        gemv_finish<double><<<(n + 255) / 256, 256, 0, s>>>(n, nch, part, alpha, beta, y);
+    }
+}
 static void zgemv(int op, int m, int n, double alpha, const c64* A, long long lda, const c64* x, double beta, c64* y,
                   hipStream_t s) {
     // timed: products of at least 2^24 entries (the operator's, not the Krylov basis')
     const bool timed = (double)m * n >= 16777216.0;
+    cx<double>* part = (cx<double>*)ws(3).get(zgemv_partial_bytes(op, m, n));
     if (timed) prof().begin(PROF_GEMV, s);
-    if (op == 0) {
-        const int chunk = 2048;
-        const int nch = (n + chunk - 1) / chunk;
-        cx<double>* part = (cx<double>*)ws(3).get((size_t)nch * m * sizeof(cx<double>));
-        gemv_n_partial<double, double><<<dim3((m + 255) / 256, nch), 256, 0, s>>>(m, n, chunk, A, lda, x, part);
-        gemv_finish<double><<<(m + 255) / 256, 256, 0, s>>>(m, nch, part, alpha, beta, y);
-    } else {
-        const int chunk = 8192;
-        const int nch = (m + chunk - 1) / chunk;
-        cx<double>* part = (cx<double>*)ws(3).get((size_t)nch * n * sizeof(cx<double>));
-        gemv_c_partial<double, double><<<dim3((n + 3) / 4, nch), 256, 0, s>>>(m, n, chunk, A, lda, x, part);
-        gemv_finish<double><<<(n + 255) / 256, 256, 0, s>>>(n, nch, part, alpha, beta, y);
-    }
+    zgemv_launch(op, m, n, alpha, A, lda, x, beta, y, part, s);
     // algorithmic bytes: A once, x and y (read + write)
     if (timed) prof().end(PROF_GEMV, s, 8.0 * m * n, 16.0 * ((double)m * n + (op == 0 ? n + 2.0 * m : m + 2.0 * n)));
     SIE_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------- GEMV without allocation
+// The partial sums of sie_zgemv_noalloc live in this statically sized device
+// array ("scratch"): each device the library runs on holds its own copy until
+// the process ends. Host code never uses the symbol itself as an address (it
+// is a host-side stand-in): the device address is looked up once per device
+// with hipGetSymbolAddress, on the first call with that device current.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+constexpr int kMaxDevices = 64;
+static void* g_scratch_ptr[kMaxDevices] = {};
+static std::mutex g_scratch_mutex;
+
+static cx<double>* scratch_of_current_device() {
+    int dev = 0;
+    SIE_HIP(hipGetDevice(&dev));
+    require(dev >= 0 && dev < kMaxDevices, "device index beyond the library's device table");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw Error(SIE_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    return (cx<double>*)g_scratch_ptr[dev];
 }
 
 // Blocked FP64 refinement of Z X = B around FP32 LU factors (op N): per sweep
@@ -3045,6 +3081,23 @@ int sie_zgemv(int op, int m, int n, double alpha, const double* A, long long lda
     require(op == 0 || op == 2, "op must be 0 (N) or 2 (C)");
     require(m > 0 && n > 0 && lda >= m, "invalid GEMV dimensions");
     zgemv(op, m, n, alpha, (const c64*)A, lda, (const c64*)x, beta, (c64*)y, (hipStream_t)stream);
+    GUARD_END
+}
+
+int sie_zgemv_noalloc(int op, int m, int n, double alpha, const double* A, long long lda, const double* x,
+                      double beta, double* y, void* stream) {
+    GUARD_BEGIN
+    require(op == 0 || op == 2, "op must be 0 (N) or 2 (C)");
+    require(m > 0 && n > 0 && lda >= m, "invalid GEMV dimensions");
+    require(A && x && y, "null GEMV argument");
+    const size_t need = zgemv_partial_bytes(op, m, n);
+    if (need > kNoAllocScratchBytes)
+        throw Error(SIE_ERR_INVALID, "sie_zgemv_noalloc: " + std::to_string(m) + " x " + std::to_string(n) +
+                                         " needs " + std::to_string(need) + " bytes of partial sums, above the " +
+                                         std::to_string(kNoAllocScratchBytes) + "-byte static scratch");
+    hipStream_t s = (hipStream_t)stream;
+    zgemv_launch(op, m, n, alpha, (const c64*)A, lda, (const c64*)x, beta, (c64*)y, scratch_of_current_device(), s);
+    SIE_HIP(hipGetLastError());
     GUARD_END
 }
 

@@ -94,6 +94,7 @@ _sig("sie_gemm", I, [I, I, I, I, D, P, LL, P, LL, D, P, LL, I, P])
 _sig("sie_lu_factor", I, [I, I, P, LL, P, ctypes.POINTER(I), I, P])
 _sig("sie_lu_solve", I, [I, I, I, I, P, LL, P, P, LL, P])
 _sig("sie_zgemv", I, [I, I, I, D, P, LL, P, D, P, P])
+_sig("sie_zgemv_noalloc", I, [I, I, I, D, P, LL, P, D, P, P])
 _sig("sie_convert", I, [I, I, LL, LL, P, LL, P, LL, D, P, P])
 _sig("sie_ir_solve", I, [I, I, I, P, LL, P, LL, P, P, LL, P, LL, D, D, I, P, P, P, P])
 _sig("sie_fgmres_ir_solve", I, [I, I, I, P, LL, P, LL, P, P, LL, P, LL, D, D, I, I, P, P, P, P])
