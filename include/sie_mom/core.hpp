@@ -1,10 +1,15 @@
-// Core types of the C++ interface over libsie_amd (MI355X). They mirror the
-// reference's host-side types (real_t, vec3_t, int2/int3 of
-// include/sie_mom/cuda/platform.cuh; IsotropicMedium, include/sie_mom/fields.hpp:13-16;
-// GstcBasisType, include/sie_mom/cuda/gstc_tensor_utils.cuh:30-33) without any
-// device-language dependency: a caller compiles against them with a plain C++17
-// compiler and links libsie_amd.so.
+// Core types of the C++ interface over libsie_amd (MI355X). The global host
+// types are those the reference's callers use unqualified (real_t, vec3_t,
+// int2, int3, complex__float and make_vec3 / make_int2 / make_int3 /
+// make_complex: the host definitions of include/sie_mom/cuda/platform.cuh in
+// its FP64 build), so a caller of the reference compiles unchanged. The
+// vector types are the HIP runtime's own (hip/hip_vector_types.h, usable from
+// any host C++ compiler with -D__HIP_PLATFORM_AMD__ and ROCm's include path),
+// so these headers and the HIP runtime's headers can meet in one translation
+// unit. The solver interfaces take std::complex<real_t>, as the reference's do.
 #pragma once
+
+#include <hip/hip_vector_types.h>
 
 #include <complex>
 #include <stdexcept>
@@ -12,27 +17,21 @@
 
 #include "sie_mom/sie_capi.h"
 
+using real_t = double;
+using vec3_t = double3;
+// complex value with the .x (real) / .y (imaginary) members of the
+// reference's host complex type
+struct complex_t {
+    real_t x, y;
+};
+using complex__float = complex_t;
+
+inline vec3_t make_vec3(real_t x, real_t y, real_t z) { return make_double3(x, y, z); }
+inline complex__float make_complex(real_t re, real_t im) { return complex__float{re, im}; }
+
 namespace sie_mom {
 
-using real_t = double;
-using complex_t = std::complex<double>;
-
-struct vec3_t {
-    double x = 0, y = 0, z = 0;
-};
-struct int2 {
-    int x = 0, y = 0;
-};
-struct int3 {
-    int x = 0, y = 0, z = 0;
-};
-inline vec3_t make_vec3(double x, double y, double z) { return vec3_t{x, y, z}; }
-
-struct IsotropicMedium {
-    real_t relative_permittivity = real_t(1);
-    real_t relative_permeability = real_t(1);
-};
-
+// include/sie_mom/cuda/gstc_tensor_utils.cuh:30-33 of the reference
 enum class GstcBasisType : int { face = 0, spherical = 1 };
 
 namespace solvers {

@@ -5,7 +5,7 @@
 
 #include <variant>
 
-#include "sie_mom/core.hpp"
+#include "sie_mom/fields.hpp"
 
 namespace sie_mom {
 namespace excitation {

@@ -28,10 +28,10 @@ public:
     int dimension() const { return dimension_; }
     // In-place LU with partial pivoting (zgetrf); dimension 0 keeps the allocated one.
     // std::runtime_error("dense matrix is singular at U diagonal k") on a zero pivot.
-    void factorize(complex_t* matrix, int dimension = 0);
+    void factorize(std::complex<real_t>* matrix, int dimension = 0);
     // op(A) X = B in place for nrhs columns (ldb 0 = dimension).
-    void solve(Op op, int nrhs, const complex_t* factored_matrix, complex_t* rhs, int ldb = 0) const;
-    void solve(const complex_t* factored_matrix, complex_t* rhs) const { solve(Op::N, 1, factored_matrix, rhs); }
+    void solve(Op op, int nrhs, const std::complex<real_t>* factored_matrix, std::complex<real_t>* rhs, int ldb = 0) const;
+    void solve(const std::complex<real_t>* factored_matrix, std::complex<real_t>* rhs) const { solve(Op::N, 1, factored_matrix, rhs); }
     const int* pivots() const { return pivots_; }
     void set_stream(void* stream) { stream_ = stream; }
 
@@ -42,7 +42,7 @@ private:
 };
 
 // Factorize and solve A x = b in place (solve_dense_lu, dense_lu_solver.cuh:189-197).
-void solve_dense_lu(int dimension, complex_t* matrix, complex_t* rhs);
+void solve_dense_lu(int dimension, std::complex<real_t>* matrix, std::complex<real_t>* rhs);
 
 // A = P L U in complex FP32 computed on the FP16 matrix cores with FP32-level
 // accuracy (each FP32 product from three FP16 products of a hi/lo split), then
@@ -57,14 +57,14 @@ public:
     MixedLuSolver(const MixedLuSolver&) = delete;
     MixedLuSolver& operator=(const MixedLuSolver&) = delete;
 
-    void factorize(const complex_t* matrix, int dimension, long long ld = 0);
+    void factorize(const std::complex<real_t>* matrix, int dimension, long long ld = 0);
     // GMRES-IR: GMRES(20) right-preconditioned by the FP32 factors (applied in
     // FP64 arithmetic) on the FP64 matrix, per column; for more than four
     // columns with op N, blocked refinement sweeps instead. maxit bounds the
     // steps (sweeps) per column. Returns the per-column normwise backward
     // errors reached (<= tol unless maxit ran out); relative_residuals
     // receives |r| / |b| per column.
-    std::vector<double> solve(Op op, int nrhs, const complex_t* rhs, int ldb, complex_t* x, int ldx,
+    std::vector<double> solve(Op op, int nrhs, const std::complex<real_t>* rhs, int ldb, std::complex<real_t>* x, int ldx,
                               double tol = 1e-15, int maxit = 30, std::vector<int>* sweeps = nullptr,
                               std::vector<double>* relative_residuals = nullptr) const;
     int dimension() const { return n_; }
@@ -75,7 +75,7 @@ public:
 private:
     int n_ = 0;
     long long ld_ = 0;
-    const complex_t* A_ = nullptr;
+    const std::complex<real_t>* A_ = nullptr;
     void* lu32_ = nullptr;
     int* pivots_ = nullptr;
     void* stream_ = nullptr;

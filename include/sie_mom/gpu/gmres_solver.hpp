@@ -1,6 +1,6 @@
 // GMRESSolver<MatVecOp> with the reference's interface (include/sie_mom/cuda/
 // gmres/gmres_solver.cuh:524-537) over sie_gmres. MatVecOp is callable as
-// op(const complex_t* x_dev, complex_t* y_dev) and writes y = A x on the device
+// op(const std::complex<real_t>* x_dev, std::complex<real_t>* y_dev) and writes y = A x on the device
 // (enqueued on the solver's stream). Semantics of solve(): as sie_gmres in
 // sie_capi.h (initial guess in d_solution, tol relative to |b|, returns 0 when
 // converged and 1 otherwise; out_res is the absolute residual estimate).
@@ -20,7 +20,7 @@ public:
     GMRESSolver(int neqns, MatVecOp matvec, void* stream = nullptr)
         : neqns_(neqns), matvec_(matvec), stream_(stream) {}
 
-    int solve(const complex_t* d_rhs, complex_t* d_solution, int m, int maxnit, real_t tol,
+    int solve(const std::complex<real_t>* d_rhs, std::complex<real_t>* d_solution, int m, int maxnit, real_t tol,
               real_t* out_res = nullptr, int* out_iters = nullptr,
               std::vector<real_t>* out_res_history = nullptr) {
         Ctx ctx{&matvec_, nullptr};
@@ -47,7 +47,7 @@ private:
     static int trampoline(void* user, const double* x, double* y, void*) {
         Ctx* c = static_cast<Ctx*>(user);
         try {
-            (*c->op)(reinterpret_cast<const complex_t*>(x), reinterpret_cast<complex_t*>(y));
+            (*c->op)(reinterpret_cast<const std::complex<real_t>*>(x), reinterpret_cast<std::complex<real_t>*>(y));
             return SIE_OK;
         } catch (...) {
             c->error = std::current_exception();
@@ -61,11 +61,11 @@ private:
 
 // y = Z x for a stored complex128 column-major matrix (the stored-Z matvec).
 struct DenseMatVec {
-    const complex_t* Z;
+    const std::complex<real_t>* Z;
     long long ldz;
     int n;
     void* stream = nullptr;
-    void operator()(const complex_t* x, complex_t* y) const {
+    void operator()(const std::complex<real_t>* x, std::complex<real_t>* y) const {
         detail::check(sie_zgemv(0, n, n, 1.0, reinterpret_cast<const double*>(Z), ldz,
                                 reinterpret_cast<const double*>(x), 0.0, reinterpret_cast<double*>(y), stream));
     }

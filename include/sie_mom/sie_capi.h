@@ -62,6 +62,15 @@ int sie_icosphere(float radius, int level, double* vertices, int* faces, int* ed
 int sie_spindle(double chord, double arc_radius, int n_rings, double* vertices, int* faces,
                 int* edges, int* face_edges, int* counts);
 
+/* Strict OBJ loader (the reference's ObjMeshLoader, src/sie_mom/mesh/
+ * obj_mesh_loader.cpp:83-386): `v` and triangular `f` records (1-based,
+ * negative relative indices and a/t/n forms), vertex and face order and
+ * winding preserved, closed-surface checks (the reference's messages, as
+ * SIE_ERR_INVALID), RWG edges in first-encounter order. Same two-call
+ * protocol and array layouts as the generators above. */
+int sie_obj_load(const char* path, double* vertices, int* faces, int* edges, int* face_edges,
+                 int* counts);
+
 /* Launch parameters of the reference assembly kernels, already rounded to
  * float exactly as the reference casts them (gstc_sie.cu:281-290). */
 typedef struct {

@@ -7,8 +7,9 @@
 #include <cstddef>
 #include <vector>
 
-#include "sie_mom/excitation.hpp"
-#include "sie_mom/mesh.hpp"
+#include "sie_mom/excitation/excitation.hpp"
+#include "sie_mom/fields.hpp"
+#include "sie_mom/mesh/spherical_surface.hpp"
 
 namespace sie_mom {
 namespace solvers {
@@ -56,7 +57,7 @@ struct DielectricMetrics {
 // physical_currents: interior-oriented [J1, M1] (M multiplied by eta0); the
 // exterior pair is [-J1, -M1].
 struct DielectricResult {
-    std::vector<complex_t> physical_currents;
+    std::vector<std::complex<real_t>> physical_currents;
     DielectricMetrics metrics;
 };
 

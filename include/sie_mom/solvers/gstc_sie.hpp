@@ -10,8 +10,9 @@
 #include <optional>
 #include <vector>
 
-#include "sie_mom/excitation.hpp"
-#include "sie_mom/mesh.hpp"
+#include "sie_mom/excitation/excitation.hpp"
+#include "sie_mom/fields.hpp"
+#include "sie_mom/mesh/spherical_surface.hpp"
 
 namespace sie_mom {
 namespace solvers {
@@ -66,7 +67,7 @@ struct GstcSieMetrics {
 // physical_currents: [J1, M1, J2, M2], one coefficient per RWG edge per block,
 // magnetic blocks multiplied by eta0 (no scaling left).
 struct GstcSieResult {
-    std::vector<complex_t> physical_currents;
+    std::vector<std::complex<real_t>> physical_currents;
     GstcSieMetrics metrics;
 };
 
@@ -103,7 +104,7 @@ struct GstcSieCaseMetrics {
 };
 
 struct GstcSieCaseResult {
-    std::vector<complex_t> physical_currents;
+    std::vector<std::complex<real_t>> physical_currents;
     GstcSieCaseMetrics metrics;
 };
 
@@ -115,12 +116,12 @@ struct GstcSieBatchResult {
 // gstc_tensors: 32 complex per face (or 32 for a uniform sheet): the 4x4
 // row-major fields-to-currents map followed by its inverse, in the frame of
 // options.basis; phasors exp(+j omega t).
-GstcSieResult solve_gstc_dense(const mesh::MeshData& mesh, const std::vector<complex_t>& gstc_tensors,
+GstcSieResult solve_gstc_dense(const mesh::MeshData& mesh, const std::vector<std::complex<real_t>>& gstc_tensors,
                                const GstcSieOptions& options, const excitation::PreparedExcitation& source);
 
 // One assembly and one factorisation shared by every case; one RHS per case
 // (the sum of its interior and exterior incident contributions).
-GstcSieBatchResult solve_gstc_dense_batch(const mesh::MeshData& mesh, const std::vector<complex_t>& gstc_tensors,
+GstcSieBatchResult solve_gstc_dense_batch(const mesh::MeshData& mesh, const std::vector<std::complex<real_t>>& gstc_tensors,
                                           const GstcSieOptions& options,
                                           const std::vector<GstcExcitationCase>& cases);
 

@@ -7,8 +7,9 @@
 #include <cstddef>
 #include <vector>
 
-#include "sie_mom/excitation.hpp"
-#include "sie_mom/mesh.hpp"
+#include "sie_mom/excitation/excitation.hpp"
+#include "sie_mom/fields.hpp"
+#include "sie_mom/mesh/spherical_surface.hpp"
 
 namespace sie_mom {
 namespace solvers {
@@ -46,8 +47,8 @@ struct PecMetrics {
 };
 
 struct PecResult {
-    std::vector<complex_t> exterior_current;   // J2 in mesh edge order
-    std::vector<complex_t> assembled_matrix;   // column-major, when captured
+    std::vector<std::complex<real_t>> exterior_current;   // J2 in mesh edge order
+    std::vector<std::complex<real_t>> assembled_matrix;   // column-major, when captured
     PecMetrics metrics;
 };
 

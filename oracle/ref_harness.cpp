@@ -17,8 +17,11 @@
 #include "sie_mom/cuda/sie_scaling.cuh"
 #include "sie_mom/quadrature/triangle_dunavant_rule.hpp"
 #include "sie_mom/mesh/spherical_surface.hpp"
+#include "sie_mom/mesh/obj_mesh_loader.hpp"
 
 #include <cstring>
+#include <exception>
+#include <string>
 #include <vector>
 
 namespace {
@@ -482,5 +485,34 @@ extern "C" void ref_fields(const double* V, const int* F, const int* FE, int nf,
             E[6 * o + 2 * x] = pe[x].x; E[6 * o + 2 * x + 1] = pe[x].y;
             H[6 * o + 2 * x] = ph[x].x; H[6 * o + 2 * x + 1] = ph[x].y;
         }
+    }
+}
+
+// The reference's ObjMeshLoader (src/sie_mom/mesh/obj_mesh_loader.cpp:83-386):
+// counts = {nv, nf, ne}; the arrays are filled when non-null. Returns 0, or 1
+// with the loader's exception message in err (errlen bytes).
+extern "C" int ref_load_obj(const char* path, int* counts, double* V, int* F, int* E, int* FE, char* err,
+                            int errlen) {
+    try {
+        sie_mom::mesh::ObjMeshLoader loader(path);
+        const auto& m = loader.getMeshData();
+        counts[0] = (int)m.vertices.size();
+        counts[1] = (int)m.faces.size();
+        counts[2] = (int)m.edges.size();
+        if (V && F && E && FE) {
+            for (size_t i = 0; i < m.vertices.size(); ++i) {
+                V[3 * i] = m.vertices[i].x; V[3 * i + 1] = m.vertices[i].y; V[3 * i + 2] = m.vertices[i].z;
+            }
+            for (size_t i = 0; i < m.faces.size(); ++i) {
+                F[3 * i] = m.faces[i].x; F[3 * i + 1] = m.faces[i].y; F[3 * i + 2] = m.faces[i].z;
+                FE[3 * i] = m.faces_edges[i].x; FE[3 * i + 1] = m.faces_edges[i].y; FE[3 * i + 2] = m.faces_edges[i].z;
+            }
+            for (size_t i = 0; i < m.edges.size(); ++i) { E[2 * i] = m.edges[i].x; E[2 * i + 1] = m.edges[i].y; }
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        std::strncpy(err, e.what(), errlen - 1);
+        err[errlen - 1] = 0;
+        return 1;
     }
 }

@@ -10,9 +10,18 @@ FLAGS=(-O3 -std=c++17 -fPIC --offload-arch=gfx950 -Xclang -target-feature -Xclan
 TMP="$HERE/_build"
 mkdir -p "$TMP"
 pids=()
-for f in "$SRC"/*.hip "$SRC"/*.cpp "$SRC"/host/*.cpp; do
+for f in "$SRC"/*.hip "$SRC"/*.cpp; do
   b=$(basename "$f")
   hipcc "${FLAGS[@]}" -c -x hip "$f" -o "$TMP/$b.o" &
+  pids+=($!)
+  if (( ${#pids[@]} >= JOBS )); then wait "${pids[0]}"; pids=("${pids[@]:1}"); fi
+done
+# the C++ interface layer is host code over the C ABI: a plain C++ compiler
+# (its public headers carry the reference's host types, which are not meant to
+# meet the HIP compiler's own vector types)
+for f in "$SRC"/host/*.cpp; do
+  b=$(basename "$f")
+  g++ -std=c++17 -O2 -fPIC -D__HIP_PLATFORM_AMD__ -I"$HERE/../include" -I/opt/rocm/include -c "$f" -o "$TMP/host_$b.o" &
   pids+=($!)
   if (( ${#pids[@]} >= JOBS )); then wait "${pids[0]}"; pids=("${pids[@]:1}"); fi
 done

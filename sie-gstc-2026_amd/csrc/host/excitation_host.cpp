@@ -3,7 +3,7 @@
 #include <limits>
 
 #include "host_common.hpp"
-#include "sie_mom/excitation.hpp"
+#include "sie_mom/excitation/excitation.hpp"
 
 namespace sie_mom {
 namespace excitation {

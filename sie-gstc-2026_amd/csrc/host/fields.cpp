@@ -1,7 +1,7 @@
 // solvers::evaluate_surface_integral_fields over the C ABI: the reference's
 // input checks (surface_current_fields.cu:46-90, 181-209) in the same order and
 // wording, then sie_surface_fields on the device.
-#include "sie_mom/fields.hpp"
+#include "sie_mom/solvers/surface_current_fields.hpp"
 
 #include <cmath>
 #include <limits>
@@ -62,7 +62,7 @@ std::vector<ElectromagneticFieldSample> evaluate_surface_integral_fields(
 
     host::MeshHandle handle(mesh);
     const std::size_t ne = mesh.edges.size(), np = observation_points.size();
-    host::DeviceArray<complex_t> J(ne), M(ne), E(3 * np), H(3 * np);
+    host::DeviceArray<std::complex<real_t>> J(ne), M(ne), E(3 * np), H(3 * np);
     host::DeviceArray<double> P(3 * np);
     J.upload(electric_currents.data(), ne);
     M.upload(magnetic_currents.data(), ne);
@@ -77,7 +77,7 @@ std::vector<ElectromagneticFieldSample> evaluate_surface_integral_fields(
                                      reinterpret_cast<const double*>(J.data()), reinterpret_cast<const double*>(M.data()),
                                      P.data(), (int)np, reinterpret_cast<double*>(E.data()),
                                      reinterpret_cast<double*>(H.data()), nullptr));
-    std::vector<complex_t> he(3 * np), hh(3 * np);
+    std::vector<std::complex<real_t>> he(3 * np), hh(3 * np);
     E.download(he.data(), he.size());
     H.download(hh.data(), hh.size());
     std::vector<ElectromagneticFieldSample> out(np);

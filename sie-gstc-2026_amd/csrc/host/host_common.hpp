@@ -14,7 +14,12 @@
 #include <vector>
 
 #include "sie_mom/core.hpp"
-#include "sie_mom/mesh.hpp"
+#include "sie_mom/mesh/spherical_surface.hpp"
+
+// the C ABI takes vertices and index triples as packed arrays
+static_assert(sizeof(vec3_t) == 3 * sizeof(double) && sizeof(int3) == 3 * sizeof(int) &&
+                  sizeof(int2) == 2 * sizeof(int),
+              "host vector types must be packed");
 
 namespace sie_mom {
 namespace host {
@@ -79,7 +84,7 @@ inline double seconds_since(Clock::time_point t0) {
 
 // Norm of a host vector accumulated in long double (the reference's vector_norm,
 // src/sie_mom/solvers/gstc_sie.cu:61-69).
-inline double host_norm(const std::vector<complex_t>& v) {
+inline double host_norm(const std::vector<std::complex<real_t>>& v) {
     long double s = 0.0L;
     for (const auto& z : v) {
         const long double a = z.real(), b = z.imag();

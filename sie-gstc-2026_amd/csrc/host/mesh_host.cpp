@@ -8,6 +8,8 @@
 #include <utility>
 
 #include "host_common.hpp"
+#include "sie_mom/mesh/circular_ogive.hpp"
+#include "sie_mom/mesh/obj_mesh_loader.hpp"
 
 namespace sie_mom {
 namespace mesh {
@@ -219,3 +221,47 @@ MeshHandle::MeshHandle(const mesh::MeshData& m) {
 }
 }  // namespace host
 }  // namespace sie_mom
+
+// C ABI of the OBJ loader (two calls, as the mesh generators: NULL arrays give
+// the counts). Defined here, with the loader, in the C++ layer of the library.
+namespace sie {
+void set_last_error(const std::string& m);  // the C ABI's thread-local message (capi.cpp)
+}
+
+extern "C" int sie_obj_load(const char* path, double* vertices, int* faces, int* edges, int* face_edges,
+                            int* counts) {
+    try {
+        if (!path || !counts) throw std::invalid_argument("null OBJ path or counts");
+        const sie_mom::mesh::MeshData m = sie_mom::mesh::load_obj(path);
+        counts[0] = (int)m.vertices.size();
+        counts[1] = (int)m.faces.size();
+        counts[2] = (int)m.edges.size();
+        if (vertices && faces && edges && face_edges) {
+            for (std::size_t i = 0; i < m.vertices.size(); ++i) {
+                vertices[3 * i] = m.vertices[i].x;
+                vertices[3 * i + 1] = m.vertices[i].y;
+                vertices[3 * i + 2] = m.vertices[i].z;
+            }
+            for (std::size_t i = 0; i < m.faces.size(); ++i) {
+                faces[3 * i] = m.faces[i].x;
+                faces[3 * i + 1] = m.faces[i].y;
+                faces[3 * i + 2] = m.faces[i].z;
+                face_edges[3 * i] = m.faces_edges[i].x;
+                face_edges[3 * i + 1] = m.faces_edges[i].y;
+                face_edges[3 * i + 2] = m.faces_edges[i].z;
+            }
+            for (std::size_t i = 0; i < m.edges.size(); ++i) {
+                edges[2 * i] = m.edges[i].x;
+                edges[2 * i + 1] = m.edges[i].y;
+            }
+        }
+        sie::set_last_error("");
+        return SIE_OK;
+    } catch (const std::invalid_argument& e) {
+        sie::set_last_error(e.what());
+        return SIE_ERR_INVALID;
+    } catch (const std::exception& e) {
+        sie::set_last_error(e.what());
+        return SIE_ERR_INVALID;
+    }
+}

@@ -1,6 +1,6 @@
 // DenseLuSolver, solve_dense_lu and MixedLuSolver over the C ABI.
 #include "host_common.hpp"
-#include "sie_mom/dense_lu_solver.hpp"
+#include "sie_mom/gpu/dense_lu_solver.hpp"
 
 namespace sie_mom {
 namespace gpu {
@@ -20,14 +20,14 @@ void DenseLuSolver::allocate(int dimension) {
     dimension_ = dimension;
 }
 
-void DenseLuSolver::factorize(complex_t* matrix, int dimension) {
+void DenseLuSolver::factorize(std::complex<real_t>* matrix, int dimension) {
     if (dimension > 0) allocate(dimension);
     if (dimension_ <= 0) throw std::invalid_argument("DenseLuSolver dimension must be initialized before factorize");
     int info = 0;
     detail::check(sie_lu_factor(64, dimension_, matrix, dimension_, pivots_, &info, 0, stream_));
 }
 
-void DenseLuSolver::solve(Op op, int nrhs, const complex_t* factored_matrix, complex_t* rhs, int ldb) const {
+void DenseLuSolver::solve(Op op, int nrhs, const std::complex<real_t>* factored_matrix, std::complex<real_t>* rhs, int ldb) const {
     if (dimension_ <= 0) throw std::invalid_argument("DenseLuSolver dimension must be initialized before solve");
     if (nrhs <= 0) throw std::invalid_argument("DenseLuSolver right-hand-side count must be positive");
     detail::check(sie_lu_solve(64, (int)op, dimension_, nrhs, factored_matrix, dimension_, pivots_, rhs,
@@ -35,7 +35,7 @@ void DenseLuSolver::solve(Op op, int nrhs, const complex_t* factored_matrix, com
     host::hip_ok(hipStreamSynchronize((hipStream_t)stream_), "LU solve");
 }
 
-void solve_dense_lu(int dimension, complex_t* matrix, complex_t* rhs) {
+void solve_dense_lu(int dimension, std::complex<real_t>* matrix, std::complex<real_t>* rhs) {
     DenseLuSolver solver(dimension);
     solver.factorize(matrix);
     solver.solve(matrix, rhs);
@@ -46,7 +46,7 @@ MixedLuSolver::~MixedLuSolver() {
     if (pivots_) (void)hipFree(pivots_);
 }
 
-void MixedLuSolver::factorize(const complex_t* matrix, int dimension, long long ld) {
+void MixedLuSolver::factorize(const std::complex<real_t>* matrix, int dimension, long long ld) {
     if (dimension <= 0) throw std::invalid_argument("MixedLuSolver dimension must be positive");
     if (ld == 0) ld = dimension;
     if (ld < dimension) throw std::invalid_argument("leading dimension below the matrix dimension");
@@ -68,7 +68,7 @@ void MixedLuSolver::factorize(const complex_t* matrix, int dimension, long long 
     detail::check(sie_lu_factor(32, dimension, lu32_, dimension, pivots_, &info, 2, stream_));
 }
 
-std::vector<double> MixedLuSolver::solve(Op op, int nrhs, const complex_t* rhs, int ldb, complex_t* x, int ldx,
+std::vector<double> MixedLuSolver::solve(Op op, int nrhs, const std::complex<real_t>* rhs, int ldb, std::complex<real_t>* x, int ldx,
                                          double tol, int maxit, std::vector<int>* sweeps,
                                          std::vector<double>* relative_residuals) const {
     if (n_ <= 0 || !A_) throw std::invalid_argument("MixedLuSolver must be factorized before solve");

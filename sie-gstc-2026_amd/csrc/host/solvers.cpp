@@ -7,10 +7,10 @@
 #include <limits>
 
 #include "host_common.hpp"
-#include "sie_mom/dense_lu_solver.hpp"
-#include "sie_mom/dielectric_solver.hpp"
-#include "sie_mom/gstc_sie.hpp"
-#include "sie_mom/pec_solver.hpp"
+#include "sie_mom/gpu/dense_lu_solver.hpp"
+#include "sie_mom/solvers/dielectric/dielectric_solver.hpp"
+#include "sie_mom/solvers/gstc_sie.hpp"
+#include "sie_mom/solvers/pec_solver.hpp"
 
 namespace sie_mom {
 namespace solvers {
@@ -60,11 +60,11 @@ void check_dimension(std::size_t dim) {
     if (dim > (std::size_t)std::numeric_limits<int>::max())
         throw std::overflow_error("SIE matrix dimension exceeds the solver int limit");
     if (dim > std::numeric_limits<std::size_t>::max() / dim) throw std::overflow_error("dense SIE matrix element count overflows size_t");
-    if (dim * dim > std::numeric_limits<std::size_t>::max() / sizeof(complex_t))
+    if (dim * dim > std::numeric_limits<std::size_t>::max() / sizeof(std::complex<real_t>))
         throw std::overflow_error("dense SIE matrix byte count overflows size_t");
 }
 
-void validate_gstc(const mesh::MeshData& mesh, const std::vector<complex_t>& tensors, const GstcSieOptions& o) {
+void validate_gstc(const mesh::MeshData& mesh, const std::vector<std::complex<real_t>>& tensors, const GstcSieOptions& o) {
     validate_common(mesh, o.frequency_hz, o.interior, o.exterior, o.vacuum_permittivity, o.vacuum_permeability,
                     o.quadrature_rule, o.singularity_threshold_lambda0);
     if (o.basis != GstcBasisType::face && o.basis != GstcBasisType::spherical)
@@ -80,7 +80,7 @@ void validate_gstc(const mesh::MeshData& mesh, const std::vector<complex_t>& ten
 
 // Per-block maxima of |coefficient| and the finiteness check of the reference
 // (validate_and_summarize_physical_currents, gstc_sie.cu:474-503).
-std::vector<double> block_maxima(const std::vector<complex_t>& c, std::size_t ne, int blocks, const char* what) {
+std::vector<double> block_maxima(const std::vector<std::complex<real_t>>& c, std::size_t ne, int blocks, const char* what) {
     std::vector<double> mx(blocks, 0.0);
     for (int b = 0; b < blocks; ++b)
         for (std::size_t e = 0; e < ne; ++e) {
@@ -93,13 +93,13 @@ std::vector<double> block_maxima(const std::vector<complex_t>& c, std::size_t ne
 }
 
 // r = Z x - b on the device, |r| on the host (long double accumulation).
-void residual(int n, const DeviceArray<complex_t>& Z, const complex_t* x_dev, const std::vector<complex_t>& b,
+void residual(int n, const DeviceArray<std::complex<real_t>>& Z, const std::complex<real_t>* x_dev, const std::vector<std::complex<real_t>>& b,
               double rhs_norm, double& abs_res, double& rel_res) {
-    DeviceArray<complex_t> r(n);
+    DeviceArray<std::complex<real_t>> r(n);
     r.upload(b.data(), n);
     detail::check(sie_zgemv(0, n, n, 1.0, reinterpret_cast<const double*>(Z.data()), n,
                             reinterpret_cast<const double*>(x_dev), -1.0, reinterpret_cast<double*>(r.data()), nullptr));
-    std::vector<complex_t> h(n);
+    std::vector<std::complex<real_t>> h(n);
     r.download(h.data(), n);
     abs_res = host::host_norm(h);
     rel_res = rhs_norm > 0 ? abs_res / rhs_norm : std::numeric_limits<double>::infinity();
@@ -107,27 +107,27 @@ void residual(int n, const DeviceArray<complex_t>& Z, const complex_t* x_dev, co
 
 struct GstcDevice {
     host::MeshHandle mesh;
-    DeviceArray<complex_t> tensors;
+    DeviceArray<std::complex<real_t>> tensors;
     sie_calderon_params params;
-    GstcDevice(const mesh::MeshData& m, const std::vector<complex_t>& t, const sie_calderon_params& p)
+    GstcDevice(const mesh::MeshData& m, const std::vector<std::complex<real_t>>& t, const sie_calderon_params& p)
         : mesh(m), tensors(t.size()), params(p) {
         tensors.upload(t.data(), t.size());
     }
-    void assemble(const GstcSieOptions& o, complex_t* Z, int n) const {
+    void assemble(const GstcSieOptions& o, std::complex<real_t>* Z, int n) const {
         detail::check(sie_assemble_gstc(mesh.get(), &params, reinterpret_cast<const double*>(tensors.data()),
                                         o.tensors_are_uniform ? 1 : 0, (int)o.basis, SIE_ASM_CALDERON | SIE_ASM_MASS,
                                         reinterpret_cast<double*>(Z), n, nullptr));
     }
     // Tested incident fields of the given waves, summed into one column.
-    void rhs(const GstcSieOptions& o, const std::vector<sie_plane_wave>& waves, complex_t* col, int n) const {
-        DeviceArray<complex_t> tmp((size_t)n * waves.size());
+    void rhs(const GstcSieOptions& o, const std::vector<sie_plane_wave>& waves, std::complex<real_t>* col, int n) const {
+        DeviceArray<std::complex<real_t>> tmp((size_t)n * waves.size());
         detail::check(sie_rhs_plane_waves(mesh.get(), SIE_RHS_GSTC, o.quadrature_rule, waves.data(), (int)waves.size(),
                                           (float)o.vacuum_permeability, (float)o.vacuum_permittivity,
                                           o.vacuum_permeability, o.vacuum_permittivity,
                                           reinterpret_cast<double*>(waves.size() == 1 ? col : tmp.data()), n, nullptr));
         if (waves.size() > 1) {
-            std::vector<complex_t> ones(waves.size(), complex_t(1.0));
-            DeviceArray<complex_t> w(ones.size());
+            std::vector<std::complex<real_t>> ones(waves.size(), std::complex<real_t>(1.0));
+            DeviceArray<std::complex<real_t>> w(ones.size());
             w.upload(ones.data(), ones.size());
             detail::check(sie_zgemv(0, n, (int)waves.size(), 1.0, reinterpret_cast<const double*>(tmp.data()), n,
                                     reinterpret_cast<const double*>(w.data()), 0.0, reinterpret_cast<double*>(col),
@@ -142,7 +142,7 @@ sie_calderon_params gstc_params(const GstcSieOptions& o, const PhysicalParameter
                        o.vacuum_permittivity, ph.omega, ph.singularity_threshold);
 }
 
-void recover(std::vector<complex_t>& c, std::size_t ne, double eta0) {
+void recover(std::vector<std::complex<real_t>>& c, std::size_t ne, double eta0) {
     for (std::size_t e = 0; e < ne; ++e) {
         c[ne + e] *= eta0;
         c[3 * ne + e] *= eta0;
@@ -153,7 +153,7 @@ void recover(std::vector<complex_t>& c, std::size_t ne, double eta0) {
 // the right-hand-side columns of X.
 class ColumnSolver {
 public:
-    ColumnSolver(const GstcSieOptions& o, DeviceArray<complex_t>& Z, int n) : o_(o), Z_(Z), n_(n) {}
+    ColumnSolver(const GstcSieOptions& o, DeviceArray<std::complex<real_t>>& Z, int n) : o_(o), Z_(Z), n_(n) {}
     void factorize() {
         if (o_.precision == SolvePrecision::fp64) {
             lu_.allocate(n_);
@@ -163,13 +163,13 @@ public:
         }
         host::hip_ok(hipDeviceSynchronize(), "LU factorisation");
     }
-    void solve(DeviceArray<complex_t>& X, int nrhs) {
+    void solve(DeviceArray<std::complex<real_t>>& X, int nrhs) {
         if (o_.precision == SolvePrecision::fp64) {
             lu_.solve(gpu::Op::N, nrhs, Z_.data(), X.data(), n_);
             return;
         }
-        DeviceArray<complex_t> B((size_t)n_ * nrhs);
-        host::hip_ok(hipMemcpy(B.data(), X.data(), sizeof(complex_t) * n_ * nrhs, hipMemcpyDeviceToDevice), "copy RHS");
+        DeviceArray<std::complex<real_t>> B((size_t)n_ * nrhs);
+        host::hip_ok(hipMemcpy(B.data(), X.data(), sizeof(std::complex<real_t>) * n_ * nrhs, hipMemcpyDeviceToDevice), "copy RHS");
         const auto be = mixed_.solve(gpu::Op::N, nrhs, B.data(), n_, X.data(), n_, o_.refinement_tolerance,
                                      o_.refinement_max_sweeps);
         for (double r : be)
@@ -180,7 +180,7 @@ public:
 
 private:
     const GstcSieOptions& o_;
-    DeviceArray<complex_t>& Z_;
+    DeviceArray<std::complex<real_t>>& Z_;
     int n_;
     gpu::DenseLuSolver lu_;
     gpu::MixedLuSolver mixed_;
@@ -201,7 +201,7 @@ PhysicalParameters make_physical_parameters(const GstcSieOptions& o) {
             o.singularity_threshold_lambda0 * c0 / o.frequency_hz};
 }
 
-GstcSieResult solve_gstc_dense(const mesh::MeshData& mesh, const std::vector<complex_t>& gstc_tensors,
+GstcSieResult solve_gstc_dense(const mesh::MeshData& mesh, const std::vector<std::complex<real_t>>& gstc_tensors,
                                const GstcSieOptions& options, const excitation::PreparedExcitation& source) {
     validate_gstc(mesh, gstc_tensors, options);
     source.validate_environment(options.frequency_hz, options.interior, options.exterior, options.vacuum_permittivity,
@@ -213,18 +213,18 @@ GstcSieResult solve_gstc_dense(const mesh::MeshData& mesh, const std::vector<com
     mt.face_count = mesh.faces.size();
     mt.edge_count = mesh.edges.size();
     mt.matrix_dimension = 4 * mesh.edges.size();
-    mt.dense_matrix_bytes = mt.matrix_dimension * mt.matrix_dimension * sizeof(complex_t);
+    mt.dense_matrix_bytes = mt.matrix_dimension * mt.matrix_dimension * sizeof(std::complex<real_t>);
     const int n = (int)mt.matrix_dimension;
     const auto ph = make_physical_parameters(options);
     auto t0 = Clock::now();
     GstcDevice dev(mesh, gstc_tensors, gstc_params(options, ph));
-    DeviceArray<complex_t> Z((size_t)n * n), x(n);
+    DeviceArray<std::complex<real_t>> Z((size_t)n * n), x(n);
     mt.device_preparation_seconds = seconds_since(t0);
     t0 = Clock::now();
     dev.assemble(options, Z.data(), n);
     dev.rhs(options, {source.plane_wave()}, x.data(), n);
     mt.assembly_seconds = seconds_since(t0);
-    std::vector<complex_t> b(n);
+    std::vector<std::complex<real_t>> b(n);
     x.download(b.data(), n);
     mt.rhs_norm = host::host_norm(b);
     if (!(mt.rhs_norm > 0.0)) throw std::runtime_error("assembled incident right-hand side has zero norm");
@@ -233,7 +233,7 @@ GstcSieResult solve_gstc_dense(const mesh::MeshData& mesh, const std::vector<com
     cs.factorize();
     cs.solve(x, 1);
     mt.solve_seconds = seconds_since(t0);
-    std::vector<complex_t> cur(n);
+    std::vector<std::complex<real_t>> cur(n);
     x.download(cur.data(), n);
     if (options.compute_residual) {
         t0 = Clock::now();
@@ -252,7 +252,7 @@ GstcSieResult solve_gstc_dense(const mesh::MeshData& mesh, const std::vector<com
     return result;
 }
 
-GstcSieBatchResult solve_gstc_dense_batch(const mesh::MeshData& mesh, const std::vector<complex_t>& gstc_tensors,
+GstcSieBatchResult solve_gstc_dense_batch(const mesh::MeshData& mesh, const std::vector<std::complex<real_t>>& gstc_tensors,
                                           const GstcSieOptions& options,
                                           const std::vector<GstcExcitationCase>& cases) {
     validate_gstc(mesh, gstc_tensors, options);
@@ -260,7 +260,7 @@ GstcSieBatchResult solve_gstc_dense_batch(const mesh::MeshData& mesh, const std:
     if (cases.empty()) throw std::invalid_argument("SIE batch must contain at least one excitation case");
     if (cases.size() > (std::size_t)std::numeric_limits<int>::max())
         throw std::overflow_error("SIE batch case count exceeds the solver int limit");
-    if (cases.size() > std::numeric_limits<std::size_t>::max() / dim / sizeof(complex_t))
+    if (cases.size() > std::numeric_limits<std::size_t>::max() / dim / sizeof(std::complex<real_t>))
         throw std::overflow_error("SIE batch RHS byte count overflows size_t");
     std::vector<std::vector<sie_plane_wave>> waves(cases.size());
     for (std::size_t i = 0; i < cases.size(); ++i) {
@@ -290,19 +290,19 @@ GstcSieBatchResult solve_gstc_dense_batch(const mesh::MeshData& mesh, const std:
     mt.face_count = mesh.faces.size();
     mt.edge_count = mesh.edges.size();
     mt.matrix_dimension = dim;
-    mt.dense_matrix_bytes = dim * dim * sizeof(complex_t);
+    mt.dense_matrix_bytes = dim * dim * sizeof(std::complex<real_t>);
     mt.case_count = cases.size();
     const int n = (int)dim, nc = (int)cases.size();
     const auto ph = make_physical_parameters(options);
     auto t0 = Clock::now();
     GstcDevice dev(mesh, gstc_tensors, gstc_params(options, ph));
-    DeviceArray<complex_t> Z((size_t)n * n), X((size_t)n * nc);
+    DeviceArray<std::complex<real_t>> Z((size_t)n * n), X((size_t)n * nc);
     mt.device_preparation_seconds = seconds_since(t0);
     t0 = Clock::now();
     dev.assemble(options, Z.data(), n);
     host::hip_ok(hipDeviceSynchronize(), "GSTC assembly");
     mt.matrix_assembly_seconds = seconds_since(t0);
-    std::vector<std::vector<complex_t>> rhs(cases.size(), std::vector<complex_t>(n));
+    std::vector<std::vector<std::complex<real_t>>> rhs(cases.size(), std::vector<std::complex<real_t>>(n));
     for (int i = 0; i < nc; ++i) {
         auto& cm = result.cases[i].metrics;
         const auto tr = Clock::now();
@@ -328,7 +328,7 @@ GstcSieBatchResult solve_gstc_dense_batch(const mesh::MeshData& mesh, const std:
     }
     for (int i = 0; i < nc; ++i) {
         auto& cr = result.cases[i];
-        std::vector<complex_t> cur(n);
+        std::vector<std::complex<real_t>> cur(n);
         X.download(cur.data(), n, (size_t)i * n);
         if (options.compute_residual) {
             const auto tr = Clock::now();
@@ -375,7 +375,7 @@ PecResult solve_pec_dense(const mesh::MeshData& mesh, const PecOptions& o, const
     mt.face_count = mesh.faces.size();
     mt.edge_count = mesh.edges.size();
     mt.matrix_dimension = mesh.edges.size();
-    mt.dense_matrix_bytes = mt.matrix_dimension * mt.matrix_dimension * sizeof(complex_t);
+    mt.dense_matrix_bytes = mt.matrix_dimension * mt.matrix_dimension * sizeof(std::complex<real_t>);
     const int n = (int)mt.matrix_dimension;
     const double omega = 2.0 * std::acos(-1.0) * o.frequency_hz;
     const double ke = omega * std::sqrt(o.exterior.relative_permeability * o.vacuum_permeability *
@@ -384,7 +384,7 @@ PecResult solve_pec_dense(const mesh::MeshData& mesh, const PecOptions& o, const
     const sie_calderon_params p = make_params(o.quadrature_rule, ke, ke, o.exterior, o.exterior, o.vacuum_permeability,
                                               o.vacuum_permittivity, omega, thr);
     host::MeshHandle mh(mesh);
-    DeviceArray<complex_t> Z((size_t)n * n), x(n);
+    DeviceArray<std::complex<real_t>> Z((size_t)n * n), x(n);
     auto t0 = Clock::now();
     auto assemble = [&] {
         detail::check(sie_assemble_pec(mh.get(), &p, reinterpret_cast<double*>(Z.data()), n, nullptr));
@@ -400,7 +400,7 @@ PecResult solve_pec_dense(const mesh::MeshData& mesh, const PecOptions& o, const
         result.assembled_matrix.resize((size_t)n * n);
         Z.download(result.assembled_matrix.data(), (size_t)n * n);
     }
-    std::vector<complex_t> b(n);
+    std::vector<std::complex<real_t>> b(n);
     x.download(b.data(), n);
     t0 = Clock::now();
     gpu::solve_dense_lu(n, Z.data(), x.data());
@@ -440,7 +440,7 @@ DielectricBatchResult solve_dielectric_dense_batch(const mesh::MeshData& mesh, c
     mt.face_count = mesh.faces.size();
     mt.edge_count = mesh.edges.size();
     mt.matrix_dimension = 2 * mesh.edges.size();
-    mt.dense_matrix_bytes = mt.matrix_dimension * mt.matrix_dimension * sizeof(complex_t);
+    mt.dense_matrix_bytes = mt.matrix_dimension * mt.matrix_dimension * sizeof(std::complex<real_t>);
     const int n = (int)mt.matrix_dimension;
     const int nrhs = (int)sources.size();
     const double omega = 2.0 * std::acos(-1.0) * o.frequency_hz;
@@ -453,7 +453,7 @@ DielectricBatchResult solve_dielectric_dense_batch(const mesh::MeshData& mesh, c
                                               o.exterior, o.vacuum_permeability, o.vacuum_permittivity, omega, thr);
     auto t0 = Clock::now();
     host::MeshHandle mh(mesh);
-    DeviceArray<complex_t> Z((size_t)n * n), X((size_t)n * nrhs);
+    DeviceArray<std::complex<real_t>> Z((size_t)n * n), X((size_t)n * nrhs);
     mt.device_preparation_seconds = seconds_since(t0);
     t0 = Clock::now();
     auto assemble = [&] {
@@ -467,7 +467,7 @@ DielectricBatchResult solve_dielectric_dense_batch(const mesh::MeshData& mesh, c
                                       o.vacuum_permittivity, reinterpret_cast<double*>(X.data()), n, nullptr));
     host::hip_ok(hipDeviceSynchronize(), "PMCHWT assembly");
     mt.assembly_seconds = seconds_since(t0);
-    std::vector<complex_t> B((size_t)n * nrhs);
+    std::vector<std::complex<real_t>> B((size_t)n * nrhs);
     X.download(B.data(), B.size());
     t0 = Clock::now();
     if (o.precision == SolvePrecision::fp64) {
@@ -477,7 +477,7 @@ DielectricBatchResult solve_dielectric_dense_batch(const mesh::MeshData& mesh, c
     } else {
         gpu::MixedLuSolver mixed;
         mixed.factorize(Z.data(), n);
-        DeviceArray<complex_t> Bd((size_t)n * nrhs);
+        DeviceArray<std::complex<real_t>> Bd((size_t)n * nrhs);
         Bd.upload(B.data(), B.size());
         const auto be = mixed.solve(gpu::Op::N, nrhs, Bd.data(), n, X.data(), n, o.refinement_tolerance,
                                     o.refinement_max_sweeps);
@@ -500,11 +500,11 @@ DielectricBatchResult solve_dielectric_dense_batch(const mesh::MeshData& mesh, c
         DielectricResult& r = batch.cases[c];
         r.metrics = mt;
         if (o.compute_residual) {
-            const std::vector<complex_t> b(B.begin() + (size_t)c * n, B.begin() + (size_t)(c + 1) * n);
+            const std::vector<std::complex<real_t>> b(B.begin() + (size_t)c * n, B.begin() + (size_t)(c + 1) * n);
             residual(n, Z, X.data() + (size_t)c * n, b, host::host_norm(b), r.metrics.absolute_residual,
                      r.metrics.relative_residual);
         }
-        std::vector<complex_t> cur(n);
+        std::vector<std::complex<real_t>> cur(n);
         X.download(cur.data(), n, (size_t)c * n);
         for (std::size_t e = 0; e < ne; ++e) cur[ne + e] *= eta0;
         const auto mx = block_maxima(cur, ne, 2, "non-finite physical current coefficient at index ");

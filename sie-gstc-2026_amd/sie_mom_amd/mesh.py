@@ -73,3 +73,13 @@ class Mesh:
         f.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int] + [ctypes.c_void_p] * 5
         f.restype = ctypes.c_int
         return Mesh._generate(f, float(chord), float(arc_radius), int(n_rings))
+
+    @staticmethod
+    def from_obj(path):
+        """The strict OBJ loader (ObjMeshLoader, obj_mesh_loader.cpp:83-386):
+        vertex and face order preserved, closed-surface checks, RWG edges in
+        first-encounter order; raises SieError with the loader's message."""
+        f = _lib.lib.sie_obj_load
+        f.argtypes = [ctypes.c_char_p] + [ctypes.c_void_p] * 5
+        f.restype = ctypes.c_int
+        return Mesh._generate(f, str(path).encode())

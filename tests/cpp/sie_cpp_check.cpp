@@ -17,13 +17,15 @@
 
 #include <hip/hip_runtime_api.h>
 
-#include "sie_mom/dense_lu_solver.hpp"
-#include "sie_mom/dielectric_solver.hpp"
-#include "sie_mom/fields.hpp"
-#include "sie_mom/gmres_solver.hpp"
-#include "sie_mom/gstc_sie.hpp"
-#include "sie_mom/io.hpp"
-#include "sie_mom/pec_solver.hpp"
+#include "sie_mom/gpu/dense_lu_solver.hpp"
+#include "sie_mom/solvers/dielectric/dielectric_solver.hpp"
+#include "sie_mom/mesh/circular_ogive.hpp"
+#include "sie_mom/mesh/obj_mesh_loader.hpp"
+#include "sie_mom/solvers/surface_current_fields.hpp"
+#include "sie_mom/gpu/gmres_solver.hpp"
+#include "sie_mom/solvers/gstc_sie.hpp"
+#include "sie_mom/gstc_utils.hpp"
+#include "sie_mom/solvers/pec_solver.hpp"
 
 using namespace sie_mom;
 
@@ -84,8 +86,8 @@ static int validate(const std::string& dir) {
     CHECK(throws<std::runtime_error>([&] { mesh::load_obj(quad); }, "only triangular faces"));
     CHECK(throws<std::runtime_error>([&] { mesh::load_obj(dir + "/missing.obj"); }, "Failed to open"));
     // tensors.bin: complex128 and complex64 forms
-    std::vector<complex_t> t(80 * 32);
-    for (std::size_t i = 0; i < t.size(); ++i) t[i] = complex_t(0.25 * i, -0.5 * i);
+    std::vector<std::complex<real_t>> t(80 * 32);
+    for (std::size_t i = 0; i < t.size(); ++i) t[i] = std::complex<real_t>(0.25 * i, -0.5 * i);
     io::write_tensors(dir + "/t64.bin", t);
     CHECK(io::read_tensors(dir + "/t64.bin", 80) == t);
     {
@@ -111,7 +113,7 @@ static int validate(const std::string& dir) {
     const auto src = excitation::prepare_source(excitation::PlaneWaveExcitation{}, f, {}, {});
     CHECK(std::abs(src.plane_wave().k - 2 * M_PI * f / 299792458.0) < 1e-6 * src.plane_wave().k);
     o.frequency_hz = f;
-    CHECK(throws<std::invalid_argument>([&] { solvers::solve_gstc_dense(sphere, std::vector<complex_t>(31), o, src); },
+    CHECK(throws<std::invalid_argument>([&] { solvers::solve_gstc_dense(sphere, std::vector<std::complex<real_t>>(31), o, src); },
                                         "GSTC tensor count mismatch: expected 2560, got 31"));
     o.quadrature_rule = 21;
     CHECK(throws<std::invalid_argument>([&] { solvers::solve_gstc_dense(sphere, t, o, src); }, "Dunavant quadrature rule"));
@@ -137,7 +139,7 @@ static int validate(const std::string& dir) {
         // field evaluation: argument checks (host side, before any device work)
     {
         using solvers::evaluate_surface_integral_fields;
-        const std::vector<complex_t> cur(sphere.edges.size(), complex_t(1, 0));
+        const std::vector<std::complex<real_t>> cur(sphere.edges.size(), std::complex<real_t>(1, 0));
         const std::vector<vec3_t> pts = {make_vec3(0, 0, 2)};
         CHECK(throws<std::invalid_argument>([&] { evaluate_surface_integral_fields(sphere, cur, cur, pts, 0.0, {}); },
                                             "field-evaluation frequency must be positive and finite"));
@@ -152,7 +154,7 @@ static int validate(const std::string& dir) {
         CHECK(throws<std::invalid_argument>([&] { evaluate_surface_integral_fields(sphere, cur, cur, nanpt, 1e9, {}); },
                                             "non-finite observation point at index 1"));
         CHECK(evaluate_surface_integral_fields(sphere, cur, cur, {}, 1e9, {}).empty());
-        const std::vector<complex_t> shortc(5);
+        const std::vector<std::complex<real_t>> shortc(5);
         CHECK(throws<std::invalid_argument>([&] { evaluate_surface_integral_fields(sphere, shortc, cur, pts, 1e9, {}); },
                                             "one electric and magnetic coefficient per mesh edge"));
     }
@@ -160,11 +162,11 @@ static int validate(const std::string& dir) {
     return failures ? 1 : 0;
 }
 
-static void dump(const std::string& path, const std::vector<complex_t>& v) { io::write_currents(path, v); }
+static void dump(const std::string& path, const std::vector<std::complex<real_t>>& v) { io::write_currents(path, v); }
 
 static int solve(const std::string& dir, double f, int rule) {
     const mesh::MeshData sphere = mesh::SphericalSurface(1.0f, 1).getMeshData();
-    const std::vector<complex_t> tensors = io::read_tensors(dir + "/tensors.bin", sphere.faces.size());
+    const std::vector<std::complex<real_t>> tensors = io::read_tensors(dir + "/tensors.bin", sphere.faces.size());
     solvers::GstcSieOptions o;
     o.frequency_hz = f;
     o.quadrature_rule = rule;
@@ -205,11 +207,11 @@ static int solve(const std::string& dir, double f, int rule) {
         // fields of the PEC currents at far, near-surface and interior points
         const std::vector<vec3_t> pts = {make_vec3(0, 0, 3), make_vec3(1.5, -0.5, 0.25), make_vec3(0.2, 0.1, -0.3),
                                          make_vec3(0, 0, 0.6), make_vec3(0.5, 0.5, 0.55)};
-        const std::vector<complex_t> zero(pec.exterior_current.size());
+        const std::vector<std::complex<real_t>> zero(pec.exterior_current.size());
         const auto smp = solvers::evaluate_surface_integral_fields(sphere, pec.exterior_current, zero, pts, f, {},
                                                                    rule, po.vacuum_permittivity,
                                                                    po.vacuum_permeability);
-        std::vector<complex_t> eh;
+        std::vector<std::complex<real_t>> eh;
         for (const auto& q : smp)
             for (int c = 0; c < 3; ++c) eh.push_back(q.electric[c]);
         for (const auto& q : smp)
@@ -270,26 +272,26 @@ static int solve(const std::string& dir, double f, int rule) {
     // operator level: GMRES on the stored PEC matrix against the LU solution
     {
         const int n = 120;
-        complex_t *Z, *b, *x;
-        if (hipMalloc((void**)&Z, sizeof(complex_t) * n * n) != 0 || hipMalloc((void**)&b, sizeof(complex_t) * n) != 0 ||
-            hipMalloc((void**)&x, sizeof(complex_t) * n) != 0)
+        std::complex<real_t> *Z, *b, *x;
+        if (hipMalloc((void**)&Z, sizeof(std::complex<real_t>) * n * n) != 0 || hipMalloc((void**)&b, sizeof(std::complex<real_t>) * n) != 0 ||
+            hipMalloc((void**)&x, sizeof(std::complex<real_t>) * n) != 0)
             throw std::runtime_error("allocation");
-        std::vector<complex_t> hb(n);
-        for (int i = 0; i < n; ++i) hb[i] = complex_t(std::cos(0.3 * i), std::sin(0.7 * i));
-        (void)hipMemcpy(Z, pec.assembled_matrix.data(), sizeof(complex_t) * n * n, hipMemcpyHostToDevice);
-        (void)hipMemcpy(b, hb.data(), sizeof(complex_t) * n, hipMemcpyHostToDevice);
-        (void)hipMemset(x, 0, sizeof(complex_t) * n);
+        std::vector<std::complex<real_t>> hb(n);
+        for (int i = 0; i < n; ++i) hb[i] = std::complex<real_t>(std::cos(0.3 * i), std::sin(0.7 * i));
+        (void)hipMemcpy(Z, pec.assembled_matrix.data(), sizeof(std::complex<real_t>) * n * n, hipMemcpyHostToDevice);
+        (void)hipMemcpy(b, hb.data(), sizeof(std::complex<real_t>) * n, hipMemcpyHostToDevice);
+        (void)hipMemset(x, 0, sizeof(std::complex<real_t>) * n);
         gpu::GMRESSolver<gpu::DenseMatVec> gm(n, gpu::DenseMatVec{Z, n, n});
         double res = 0;
         int its = 0;
         std::vector<double> hist;
         const int info = gm.solve(b, x, 60, 5, 1e-12, &res, &its, &hist);
         CHECK(info == 0 && its > 0 && (int)hist.size() == its);
-        std::vector<complex_t> xg(n), xl = hb;
-        (void)hipMemcpy(xg.data(), x, sizeof(complex_t) * n, hipMemcpyDeviceToHost);
-        (void)hipMemcpy(b, hb.data(), sizeof(complex_t) * n, hipMemcpyHostToDevice);
+        std::vector<std::complex<real_t>> xg(n), xl = hb;
+        (void)hipMemcpy(xg.data(), x, sizeof(std::complex<real_t>) * n, hipMemcpyDeviceToHost);
+        (void)hipMemcpy(b, hb.data(), sizeof(std::complex<real_t>) * n, hipMemcpyHostToDevice);
         gpu::solve_dense_lu(n, Z, b);
-        (void)hipMemcpy(xl.data(), b, sizeof(complex_t) * n, hipMemcpyDeviceToHost);
+        (void)hipMemcpy(xl.data(), b, sizeof(std::complex<real_t>) * n, hipMemcpyDeviceToHost);
         double num = 0, den = 0;
         for (int i = 0; i < n; ++i) {
             num += std::norm(xg[i] - xl[i]);

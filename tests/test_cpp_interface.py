@@ -20,6 +20,17 @@ def _binary():
     return CHK
 
 
+def test_reference_caller_compiles():
+    """tests/cpp/contract_check.cpp, a caller written against the reference's
+    include paths, namespaces, global host types and signatures, compiles
+    against include/ with a plain C++17 compiler."""
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-D__HIP_PLATFORM_AMD__",
+                        "-I" + os.path.join(ROOT, "include"), "-I/opt/rocm/include",
+                        os.path.join(ROOT, "tests", "cpp", "contract_check.cpp")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+
+
 def test_cpp_validation_and_formats(tmp_path):
     r = subprocess.run([_binary(), "validate", str(tmp_path)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
