@@ -1396,6 +1396,18 @@ __global__ __launch_bounds__(256) void trsm_lanes_kernel(int w, const cx<T>* __r
 }
 
 // ================================================================= GEMV
+// Streaming (nontemporal) load of one complex entry of an operator read once.
+typedef double double2v __attribute__((ext_vector_type(2)));
+typedef float float2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ cx<double> ld_stream(const cx<double>* p) {
+    const double2v v = __builtin_nontemporal_load(reinterpret_cast<const double2v*>(p));
+    return {v[0], v[1]};
+}
+__device__ __forceinline__ cx<float> ld_stream(const cx<float>* p) {
+    const float2v v = __builtin_nontemporal_load(reinterpret_cast<const float2v*>(p));
+    return {v[0], v[1]};
+}
+
 // Partial y_part[chunk][i] = sum_{j in chunk} A(i, j) x(j), op N.
 template <typename T, typename TX>
 __global__ void gemv_n_partial(int m, int n, int chunk, const cx<T>* __restrict__ A, long long lda,
@@ -1404,16 +1416,31 @@ __global__ void gemv_n_partial(int m, int n, int chunk, const cx<T>* __restrict_
     const int c = blockIdx.y;
     const int j0 = c * chunk, j1 = min(n, j0 + chunk);
     if (i >= m) return;
-    double sr = 0, si = 0;
+    // eight column loads in flight per lane, two independent accumulator
+    // pairs (even / odd columns) combined at the end in a fixed order
+    double sr0 = 0, si0 = 0, sr1 = 0, si1 = 0;
     const cx<T>* a = A + (long long)j0 * lda + i;
-#pragma unroll 4
-    for (int j = j0; j < j1; ++j, a += lda) {
+    int j = j0;
+    for (; j + 8 <= j1; j += 8, a += 8 * lda) {
+        cx<T> v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = ld_stream(a + u * lda);
+#pragma unroll
+        for (int u = 0; u < 8; u += 2) {
+            const cx<TX> x0 = x[j + u], x1 = x[j + u + 1];
+            sr0 += (double)v[u].re * x0.re - (double)v[u].im * x0.im;
+            si0 += (double)v[u].re * x0.im + (double)v[u].im * x0.re;
+            sr1 += (double)v[u + 1].re * x1.re - (double)v[u + 1].im * x1.im;
+            si1 += (double)v[u + 1].re * x1.im + (double)v[u + 1].im * x1.re;
+        }
+    }
+    for (; j < j1; ++j, a += lda) {
         const cx<T> v = *a;
         const cx<TX> xv = x[j];
-        sr += (double)v.re * xv.re - (double)v.im * xv.im;
-        si += (double)v.re * xv.im + (double)v.im * xv.re;
+        sr0 += (double)v.re * xv.re - (double)v.im * xv.im;
+        si0 += (double)v.re * xv.im + (double)v.im * xv.re;
     }
-    part[(size_t)c * m + i] = {sr, si};
+    part[(size_t)c * m + i] = {sr0 + sr1, si0 + si1};
 }
 
 // op C: y_part[chunk][j] = sum_{i in chunk} conj(A(i, j)) x(i); one wave per column.
@@ -1425,9 +1452,25 @@ __global__ void gemv_c_partial(int m, int n, int chunk, const cx<T>* __restrict_
     const int c = blockIdx.y;
     if (j >= n) return;
     const int i0 = c * chunk, i1 = min(m, i0 + chunk);
+    // four 1 KB row segments in flight per wave, accumulated in order
     double sr = 0, si = 0;
     const cx<T>* a = A + (long long)j * lda;
-    for (int i = i0 + lane; i < i1; i += 64) {
+    int i = i0 + lane;
+    for (; i + 192 < i1; i += 256) {
+        cx<T> v[4];
+        cx<TX> xv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            v[u] = ld_stream(a + i + 64 * u);
+            xv[u] = x[i + 64 * u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            sr += (double)v[u].re * xv[u].re + (double)v[u].im * xv[u].im;
+            si += (double)v[u].re * xv[u].im - (double)v[u].im * xv[u].re;
+        }
+    }
+    for (; i < i1; i += 64) {
         const cx<T> v = a[i];
         const cx<TX> xv = x[i];
         sr += (double)v.re * xv.re + (double)v.im * xv.im;
