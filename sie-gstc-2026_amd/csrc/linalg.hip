@@ -28,6 +28,7 @@
 #include <map>
 #include <mutex>
 #include <cmath>
+#include <cstdlib>
 #include <type_traits>
 
 namespace sie {
@@ -973,14 +974,47 @@ __global__ void fill_unsolved_kernel(long long nwords, unsigned long long* __res
 // launch and column. A reader that sees every word carry its tag has the
 // whole row, so the writer need not wait for its stores to complete before
 // publishing its key: the reader re-reads the (rare) row not yet complete.
-__device__ __forceinline__ void publish_row_tagged(unsigned long long* __restrict__ dst, const cx<float>* a, int W,
-                                                   unsigned tag) {
-    const unsigned long long hi = (unsigned long long)tag << 32;
-    for (int k = 0; k < W; ++k) {
-        __hip_atomic_store(dst + 2 * k, hi | __float_as_uint(a[k].re), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(dst + 2 * k + 1, hi | __float_as_uint(a[k].im), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+// One wave publishes a row that one of its lanes holds (mine): the row goes
+// through the wave's LDS stage and each lane stores one word.
+template <int W>
+__device__ __forceinline__ void publish_row_tagged(float* __restrict__ stage, const cx<float>* a, bool mine,
+                                                   unsigned long long* __restrict__ dst, unsigned tag) {
+    if (mine) {
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            stage[2 * k] = a[k].re;
+            stage[2 * k + 1] = a[k].im;
+        }
     }
+    // the same wave reads what its lane wrote
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int lane = threadIdx.x & 63;
+    if (lane < 2 * W)
+        __hip_atomic_store(dst + lane, ((unsigned long long)tag << 32) | __float_as_uint(stage[lane]),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Maximum of a 32-bit unsigned value over the wave, returned to every lane:
+// DPP steps within rows of 16 lanes (quad permutations, half-row and row
+// mirrors), then the broadcasts of lane 15 and lane 31 into the rows above;
+// lane 63 ends with the maximum.
+__device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
+    unsigned t;
+    t = (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+    v = v > t ? v : t;
+    t = (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+    v = v > t ? v : t;
+    t = (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x141, 0xF, 0xF, false);  // row_half_mirror
+    v = v > t ? v : t;
+    t = (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x140, 0xF, 0xF, false);  // row_mirror
+    v = v > t ? v : t;
+    t = (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x142, 0xA, 0xF, false);  // row_bcast:15
+    v = v > t ? v : t;
+    t = (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x143, 0xC, 0xF, false);  // row_bcast:31
+    v = v > t ? v : t;
+    return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
 }
 
 // One wave reads a tagged row of W <= 32 entries (2W words, one per lane)
@@ -1005,15 +1039,44 @@ __device__ __forceinline__ void read_row_tagged(const unsigned long long* __rest
     if (lane < 2 * W) out[lane] = __uint_as_float((unsigned)v);
 }
 
-// The FP32 panel with the candidates as keys: per column, each workgroup
-// publishes its candidate row (tagged words), then its key into the column's
-// own slot; every workgroup polls the G keys of the column until all are
-// published (this wait is the grid-wide synchronisation: no counter), takes
-// the largest and reads that row and row j, whose tags prove them complete.
-// Rows, keys and their reads are agent-scope (device-coherent) accesses, so
-// no L2 writeback or invalidate is needed per column. Slots are per column
-// and tags per launch, so no stale word is taken for a current one. Requires
-// all G workgroups co-resident (cooperative launch).
+// The same for two rows at once (2W words each, two per lane), both carrying
+// the same tag: one round trip for the pivot row and row j.
+__device__ __forceinline__ void read_rows_tagged(const unsigned long long* __restrict__ src1,
+                                                 const unsigned long long* __restrict__ src2, int W, unsigned tag,
+                                                 float* __restrict__ out1, float* __restrict__ out2,
+                                                 unsigned int* err) {
+    const int lane = threadIdx.x & 63;
+    unsigned long long v1 = 0, v2 = 0;
+    bool complete = false;
+    for (unsigned it = 0; it < (1u << 20); ++it) {
+        const unsigned long long mark = (unsigned long long)tag << 32;
+        v1 = lane < 2 * W ? __hip_atomic_load(src1 + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : mark;
+        v2 = lane < 2 * W ? __hip_atomic_load(src2 + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : mark;
+        if (__all((unsigned)(v1 >> 32) == tag && (unsigned)(v2 >> 32) == tag)) {
+            complete = true;
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    if (!complete && lane == 0) flag_wait_error(err, kWaitRow);
+    if (lane < 2 * W) {
+        out1[lane] = __uint_as_float((unsigned)v1);
+        out2[lane] = __uint_as_float((unsigned)v2);
+    }
+}
+
+// The FP32 panel with the candidates as keys: per column, each wave finds its
+// candidate (DPP maximum), the workgroup's candidate is the best of its waves;
+// each workgroup publishes its candidate row (tagged words), then its key into
+// the column's own slot, and the owner of row j publishes row j. Wave 0 polls
+// the G keys of the column until all are published (this wait is the
+// grid-wide synchronisation: no counter), takes the largest and reads that
+// row and row j in one round trip; the tags prove them complete. Rows, keys and
+// their reads are agent-scope (device-coherent) accesses, so no L2 writeback
+// or invalidate is needed per column. Slots are per column and tags per
+// launch, so no stale word is taken for a current one. Three workgroup
+// barriers per column. Requires all G workgroups co-resident (cooperative
+// launch).
 template <int W, int BS>
 __global__ __launch_bounds__(BS) void panel_getf2_keyed_kernel(int M, int w, cx<float>* __restrict__ P,
                                                                long long lda, int* __restrict__ ipiv, int ipiv_off,
@@ -1043,13 +1106,15 @@ __global__ __launch_bounds__(BS) void panel_getf2_keyed_kernel(int M, int w, cx<
     __shared__ int swin;
     __shared__ cx<float> sprow[W];
     __shared__ cx<float> sdrow[W];
+    __shared__ float stage_c[2 * W], stage_j[2 * W];  // rows on their way out
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 
     for (int j = 0; j < w; ++j) {
         const unsigned tag = tag_base | (unsigned)(j + 1);
         const int jg = j >> 3;  // group of 8 columns holding column j (uniform)
-        // 1. local argmax over rows >= j of column j (static register indexing
-        // within the group of column j)
+        // 1. each wave's candidate over rows >= j of column j (static register
+        // indexing within the group of column j): the largest |re| + |im| as
+        // its bits + 1 (0: no row >= j in the wave), the lowest row among equals
         cx<float> aj = {0, 0};
 #pragma unroll
         for (int g = 0; g < NG; ++g)
@@ -1058,31 +1123,33 @@ __global__ __launch_bounds__(BS) void panel_getf2_keyed_kernel(int M, int w, cx<
                 for (int kk = 0; kk < 8; ++kk)
                     if (8 * g + kk == j) aj = a[8 * g + kk];
             }
-        unsigned long long key = pivot_key(0.f, 0x7fffffff);
-        if (own && r >= j) key = pivot_key(cabs1(aj), r);
-        for (int off = 32; off > 0; off >>= 1) {
-            const unsigned long long o = __shfl_xor(key, off);
-            key = o > key ? o : key;
-        }
-        if (lane == 0) skey[wv] = key;
+        const unsigned enc = (own && r >= j) ? __float_as_uint(cabs1(aj)) + 1u : 0u;
+        const unsigned m = wave_max_u32(enc);
+        const int L = __ffsll((long long)__ballot(enc == m)) - 1;
+        if (lane == 0)
+            skey[wv] = m != 0u ? pivot_key(__uint_as_float(m - 1u), blockIdx.x * BS + wv * 64 + L)
+                               : pivot_key(0.f, 0x7fffffff);
         __syncthreads();
-        if (threadIdx.x == 0) {
-            unsigned long long b = skey[0];
-            for (int q = 1; q < BS / 64; ++q) b = skey[q] > b ? skey[q] : b;
-            skey[0] = b;
-            swin = (int)(0xFFFFFFFFu - (unsigned)(b & 0xFFFFFFFFu));
-        }
-        __syncthreads();
-        // 2. publish the local winner's row and, by its owner, row j; then the key
-        if (own && r == swin) publish_row_tagged(pub + ((size_t)j * G + blockIdx.x) * 2 * W, a, W, tag);
-        if (own && r == j) publish_row_tagged(drow + (size_t)j * 2 * W, a, W, tag);
+        // 2. every wave takes the workgroup's candidate (the largest key);
+        // thread 0 publishes the key, the holders of the candidate row and of
+        // row j publish those rows
+        unsigned long long b = skey[0];
+#pragma unroll
+        for (int q = 1; q < BS / 64; ++q) b = skey[q] > b ? skey[q] : b;
         if (threadIdx.x == 0)
-            __hip_atomic_store(&keys[(size_t)j * G + blockIdx.x], skey[0], __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        // 3. wait for every workgroup's key of column j; the largest wins.
+            __hip_atomic_store(&keys[(size_t)j * G + blockIdx.x], b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // (measured: polling that starts while the other waves publish rows
+        // makes the column 0.25 us slower)
+        __syncthreads();
+        const int lw = (int)(0xFFFFFFFFu - (unsigned)(b & 0xFFFFFFFFu));
+        const bool mc = own && r == lw, mj = own && r == j;
+        if (__any(mc)) publish_row_tagged<W>(stage_c, a, mc, pub + ((size_t)j * G + blockIdx.x) * 2 * W, tag);
+        if (__any(mj)) publish_row_tagged<W>(stage_j, a, mj, drow + (size_t)j * 2 * W, tag);
+        // 3. wave 0 waits for every workgroup's key of column j, takes the
+        // largest and reads that row together with row j.
         // G <= kPanelKeyedMaxG: each lane polls its (up to) 4 keys with the
         // loads of one round in flight together
-        if (threadIdx.x < 64) {
+        if (wv == 0) {
             unsigned long long best = 0;
             int bg = 0;
             bool stalled = false;
@@ -1113,19 +1180,24 @@ __global__ __launch_bounds__(BS) void panel_getf2_keyed_kernel(int M, int w, cx<
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
-            for (int off = 32; off > 0; off >>= 1) {
-                const unsigned long long ob = __shfl_xor(best, off);
-                const int og = __shfl_xor(bg, off);
-                if (ob > best) { best = ob; bg = og; }
+            // the largest key of the wave: its high word, then the low word
+            // among the lanes holding that high word (keys with a nonzero
+            // high word are distinct)
+            const unsigned hi = (unsigned)(best >> 32), lo = (unsigned)best;
+            const unsigned mh = wave_max_u32(hi);
+            const unsigned ml = wave_max_u32(hi == mh ? lo : 0u);
+            const int lb = __ffsll((long long)__ballot(hi == mh && lo == ml)) - 1;
+            bg = __builtin_amdgcn_readlane(bg, lb);
+            const int bi = (int)(0xFFFFFFFFu - ml);
+            int code = mh != 0u ? (bi | (bg << 20)) : -1;
+            if (stalled) {
+                if (lane == 0) flag_wait_error(wa.err, kWaitPanel);
+                code = kStalled;
             }
-            if (threadIdx.x == 0) {
-                const int bi = (int)(0xFFFFFFFFu - (unsigned)(best & 0xFFFFFFFFu));
-                swin = (best >> 32) != 0 ? (bi | (bg << 20)) : -1;
-                if (stalled) {
-                    flag_wait_error(wa.err, kWaitPanel);
-                    swin = kStalled;
-                }
-            }
+            if (code >= 0)
+                read_rows_tagged(pub + ((size_t)j * G + bg) * 2 * W, drow + (size_t)j * 2 * W, W, tag,
+                                 (float*)sprow, (float*)sdrow, wa.err);
+            if (lane == 0) swin = code;
         }
         __syncthreads();
         const int code = swin;
@@ -1138,11 +1210,7 @@ __global__ __launch_bounds__(BS) void panel_getf2_keyed_kernel(int M, int w, cx<
             }
             continue;
         }
-        const int p = code & 0xFFFFF, pg = code >> 20;
-        // the pivot row (wave 0) and row j (wave 1), shared by the workgroup
-        if (wv == 0) read_row_tagged(pub + ((size_t)j * G + pg) * 2 * W, W, tag, (float*)sprow, wa.err);
-        else if (wv == 1) read_row_tagged(drow + (size_t)j * 2 * W, W, tag, (float*)sdrow, wa.err);
-        __syncthreads();
+        const int p = code & 0xFFFFF;
         if (blockIdx.x == 0 && threadIdx.x == 0) ipiv[j] = p + ipiv_off;
         // swap: position p takes old row j, position j takes the pivot row
         if (own && r == p && p != j) {
@@ -2515,11 +2583,20 @@ __global__ void put_rows_kernel(cx<T>* __restrict__ A, long long lda, int c0, in
 // after the factorisation: the columns of block L (kLevels[0] wide) receive
 // the swaps of every later block as one row permutation of rows [end of
 // block L, n), composed on the host from ipiv, gathered into a temporary (at
-// most kDeferredTmpBytes of columns at a time) and copied back. Rows are only
-// moved, so the factors are those of applying each block's swaps as it is
+// most deferred_tmp_bytes() of columns at a time) and copied back. Rows are
+// only moved, so the factors are those of applying each block's swaps as it is
 // factored; the 4096 scattered swaps per column and block become one
 // coalesced pass per column.
 constexpr size_t kDeferredTmpBytes = size_t(1) << 30;
+
+// The temporary's size: kDeferredTmpBytes, or SIE_DEFERRED_TMP_BYTES (at least
+// 4096) when set, so that tests reach the chunked loop at small n.
+static size_t deferred_tmp_bytes() {
+    const char* e = std::getenv("SIE_DEFERRED_TMP_BYTES");
+    if (e == nullptr) return kDeferredTmpBytes;
+    const long long v = std::atoll(e);
+    return v >= 4096 ? (size_t)v : kDeferredTmpBytes;
+}
 
 template <typename T>
 static void apply_deferred_swaps(int n, cx<T>* A, long long lda, const int* ipiv, hipStream_t s) {
@@ -2544,14 +2621,15 @@ static void apply_deferred_swaps(int n, cx<T>* A, long long lda, const int* ipiv
         }
     }
     const size_t src_bytes = (src.size() * sizeof(int) + 255) / 256 * 256;
-    unsigned char* w = (unsigned char*)ws(7).get(src_bytes + kDeferredTmpBytes);
+    const size_t tmp_bytes = deferred_tmp_bytes();
+    unsigned char* w = (unsigned char*)ws(7).get(src_bytes + tmp_bytes);
     int* dsrc = (int*)w;
     cx<T>* tmp = (cx<T>*)(w + src_bytes);
     SIE_HIP(hipMemcpyAsync(dsrc, src.data(), src.size() * sizeof(int), hipMemcpyHostToDevice, s));
     SIE_HIP(hipStreamSynchronize(s));
     for (size_t L = 0; L < off.size(); ++L) {
         const int c = (int)L * bw, a = c + bw, m = n - a;
-        const int chunk = (int)std::max<size_t>(1, std::min<size_t>(bw, kDeferredTmpBytes / ((size_t)m * sizeof(cx<T>))));
+        const int chunk = (int)std::max<size_t>(1, std::min<size_t>(bw, tmp_bytes / ((size_t)m * sizeof(cx<T>))));
         for (int c0 = c; c0 < a; c0 += chunk) {
             const int nc = std::min(chunk, a - c0);
             const dim3 g((unsigned)((m + 255) / 256), (unsigned)nc);

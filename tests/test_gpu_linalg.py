@@ -92,6 +92,43 @@ def test_lu_solve(sm, prec, variant, n):
     assert np.all(np.abs(L) <= np.sqrt(2) * (1 + 1e-6))
 
 
+@pytest.mark.parametrize("prec,variant", [(64, "4m"), (32, "f16x3")])
+def test_lu_deferred_swaps(sm, prec, variant, monkeypatch):
+    """n = 12,500: the 4096-column blocks at 0, 4096 and 8192 each receive the
+    swaps of every later block after the factorisation (three deferred
+    permutations, composed from up to three later blocks), moved through a
+    1 MiB temporary so that each block's columns go in many chunks. Checked on
+    the GPU: ||PA - LU|| / ||A|| and the backward error of a solve."""
+    n = 12500
+    monkeypatch.setenv("SIE_DEFERRED_TMP_BYTES", str(1 << 20))
+    dt = torch.complex128 if prec == 64 else torch.complex64
+    g = torch.Generator(device="cuda").manual_seed(12500)
+    A = torch.randn(n, n, dtype=dt, device="cuda", generator=g).T  # column-major view
+    Ad = A.clone()
+    assert Ad.stride() == (1, n)
+    print(f"lu deferred swaps prec={prec} {variant} n={n}", flush=True)
+    lu = sm.linalg.DenseLuSolver(variant=variant)
+    lu.factorize(Ad)
+    piv = lu.pivots.cpu().numpy()
+    assert np.all((piv >= np.arange(n)) & (piv < n))
+    perm = np.arange(n)
+    for k in range(n):
+        p = piv[k]
+        perm[k], perm[p] = perm[p], perm[k]
+    L = torch.tril(Ad, -1) + torch.eye(n, dtype=dt, device="cuda")
+    U = torch.triu(Ad)
+    PA = A[torch.from_numpy(perm).cuda()]
+    err = (torch.linalg.norm(PA - L @ U) / torch.linalg.norm(A)).item()
+    del L, U, PA
+    eps = 1.2e-16 if prec == 64 else 6e-8
+    assert err < 4 * n * eps, err
+    b = torch.randn(n, 1, dtype=dt, device="cuda", generator=g)
+    x = b.clone()
+    lu.solve(x)
+    be = (torch.linalg.norm(A @ x - b) / (torch.linalg.norm(A) * torch.linalg.norm(x))).item()
+    assert be < (1e-14 if prec == 64 else 5e-6), be
+
+
 def test_singular_reported(sm):
     n = 64
     rng = np.random.default_rng(1)
