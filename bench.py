@@ -597,6 +597,18 @@ def main():
             "traffic": traffic_per_launch(dom), "avg_ms": r.get("avg_ms"),
             "launches": r.get("launches"), "alg_bytes_per_launch": r.get("alg_bytes_per_launch")}
 
+    # the FP16 matrix cores' sustained rate on this device, under the GEMM's
+    # instruction mix, and the clock held meanwhile: the attainable ceiling
+    # behind the published peak (which assumes the 2.4 GHz peak clock)
+    if roof["kernel"] in ("gemm_f16x3", "gemm_f16x3_wide") and roof["achieved"]:
+        tf16, mhz = sm.linalg.mfma_f16_rate(iters=10000, reps=3)
+        roof["sustained"] = {"mfma_f16_tflops": tf16, "clock_mhz": mhz, "peak": tf16 / 3,
+                             "frac": roof["achieved"] / (tf16 / 3),
+                             "method": "48 independent 32x32x16 FP16 MFMAs per wave and slice from registers, "
+                                       "one 512-thread workgroup per CU; best of 3 launches"}
+        log(f"sustained FP16 MFMA rate {tf16:.0f} TFLOP/s at {mhz:.0f} MHz: split-FP16 ceiling {tf16 / 3:.0f} "
+            f"TFLOP/s, wide GEMM at {roof['achieved'] / (tf16 / 3):.2f} of it")
+
     cpu = None
     if not args.no_cpu:
         try:

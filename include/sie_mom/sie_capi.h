@@ -279,6 +279,15 @@ int sie_release(void);
  * No counterpart in the reference. */
 int sie_set_wait_bound(double seconds, int force_timeout);
 
+/* Sustained rate of the FP16 matrix cores on the current device under the
+ * split-FP16 GEMM's instruction mix (register operands, no memory traffic):
+ * one 512-thread workgroup per CU runs `iters` slices of 48 32x32x16 FP16
+ * MFMAs per wave, `reps` times; *tflops receives the best rate (FP16 MFMA
+ * flops, 2 M N K per instruction) and *clock_mhz the shader clock held during
+ * that launch (s_memtime cycles over 100 MHz s_memrealtime ticks). A
+ * measurement for rooflines; no counterpart in the reference. Synchronises. */
+int sie_mfma_f16_rate(int iters, int reps, double* tflops, double* clock_mhz, void* stream);
+
 /* Euclidean norm of n complex doubles (deterministic two-stage reduction,
  * synchronises the stream; returns -1 on error). */
 double sie_norm2(long long n, const double* x_dev, void* stream);

@@ -700,6 +700,61 @@ __global__ __launch_bounds__(512, 1) void cgemm_f16x3_r3_kernel(int M, int N, in
     f16x3_acc_store<TI>(accR, accI, C, ldc, M, N, i0 + wm * 64, j0 + wn * 64, lane & 31, lane >> 5, alpha / unit);
 }
 
+// Sustained rate of the FP16 matrix cores under the split-FP16 GEMM's
+// instruction mix: every wave runs the per-slice product of that GEMM's wave
+// tile (48 independent-accumulator 32x32x16 FP16 MFMAs with their operand
+// negations and unshifts) on register operands, with no memory traffic, so
+// the rate is what the matrix cores deliver at the clock the device holds
+// under this load. Operands are pseudo-random FP16 of modulus < 1 (the bit
+// activity of real split planes). Wave 0 of each workgroup records its shader
+// clock cycles (s_memtime) and 100 MHz constant-clock ticks (s_memrealtime).
+__global__ __launch_bounds__(512, 1) void mfma_f16_rate_kernel(int iters, float* __restrict__ sink,
+                                                               unsigned long long* __restrict__ clocks) {
+    const int tid = threadIdx.x;
+    unsigned h = 2654435761u * (unsigned)(blockIdx.x * 512 + tid + 1);
+    auto next = [&]() {
+        h ^= h << 13;
+        h ^= h >> 17;
+        h ^= h << 5;
+        return (_Float16)((float)(h & 0xFFFF) * (1.f / 65536.f) - 0.5f);
+    };
+    half8 a[2][4], b[2][4];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                a[t][q][e] = next();
+                b[t][q][e] = next();
+            }
+    floatx16 accR[2][2], accI[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) accR[i][j][v] = accI[i][j][v] = 0.f;
+    const unsigned long long c0 = __builtin_amdgcn_s_memtime(), r0 = __builtin_amdgcn_s_memrealtime();
+    for (int it = 0; it < iters; ++it) {
+        f16x3_mfma<2>(a, b, accR, accI);
+        asm volatile("" ::: "memory");
+    }
+    const unsigned long long c1 = __builtin_amdgcn_s_memtime(), r1 = __builtin_amdgcn_s_memrealtime();
+    float sum = 0.f;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) sum += accR[i][j][v] + accI[i][j][v];
+    sink[blockIdx.x * 512 + tid] = sum;
+    if (tid == 0) {
+        clocks[2 * blockIdx.x] = c1 - c0;
+        clocks[2 * blockIdx.x + 1] = r1 - r0;
+    }
+}
+
 // =================================================================== panel
 // Grid barrier over `nblocks` co-resident workgroups: monotone counter.
 // Bounded (WaitArg): returns true, uniformly over the workgroup, when the
@@ -3319,6 +3374,50 @@ double sie_norm2(long long n, const double* x, void* stream) {
     } catch (...) {
         return -1.0;
     }
+}
+
+int sie_mfma_f16_rate(int iters, int reps, double* tflops, double* clock_mhz, void* stream) {
+    GUARD_BEGIN
+    require(iters > 0 && reps > 0 && tflops && clock_mhz, "invalid calibration arguments");
+    hipStream_t s = (hipStream_t)stream;
+    int dev = 0, ncu = 0;
+    SIE_HIP(hipGetDevice(&dev));
+    SIE_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const int grid = ncu;  // one 512-thread workgroup per CU (its registers allow no second)
+    DevBuf<float> sink((size_t)grid * 512);
+    DevBuf<unsigned long long> clocks((size_t)grid * 2);
+    hipEvent_t e0, e1;
+    SIE_HIP(hipEventCreate(&e0));
+    SIE_HIP(hipEventCreate(&e1));
+    double best = 0.0, mhz = 0.0;
+    std::vector<unsigned long long> hc((size_t)grid * 2);
+    for (int r = 0; r < reps; ++r) {
+        SIE_HIP(hipEventRecord(e0, s));
+        mfma_f16_rate_kernel<<<grid, 512, 0, s>>>(iters, sink.p, clocks.p);
+        SIE_HIP(hipGetLastError());
+        SIE_HIP(hipEventRecord(e1, s));
+        SIE_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        SIE_HIP(hipEventElapsedTime(&ms, e0, e1));
+        // 48 MFMAs of 2 * 32 * 32 * 16 flops per wave and iteration, 8 waves
+        const double fl = (double)grid * 8 * iters * 48 * 32768.0;
+        const double tf = fl / (ms * 1e-3) / 1e12;
+        if (tf > best) {
+            best = tf;
+            SIE_HIP(hipMemcpy(hc.data(), clocks.p, hc.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+            double cyc = 0.0, ticks = 0.0;
+            for (int b = 0; b < grid; ++b) {
+                cyc += (double)hc[2 * b];
+                ticks += (double)hc[2 * b + 1];
+            }
+            mhz = ticks > 0.0 ? cyc / ticks * 100.0 : 0.0;
+        }
+    }
+    SIE_HIP(hipEventDestroy(e0));
+    SIE_HIP(hipEventDestroy(e1));
+    *tflops = best;
+    *clock_mhz = mhz;
+    GUARD_END
 }
 
 }  // extern "C"

@@ -101,6 +101,7 @@ _sig("sie_fgmres_ir_solve", I, [I, I, I, P, LL, P, LL, P, P, LL, P, LL, D, D, I,
 _sig("sie_set_wait_bound", I, [D, I])
 _sig("sie_release", I, [])
 _sig("sie_norm2", D, [LL, P, P])
+_sig("sie_mfma_f16_rate", I, [I, I, ctypes.POINTER(D), ctypes.POINTER(D), P])
 MATVEC_FN = ctypes.CFUNCTYPE(I, P, P, P, P)
 _sig("sie_gmres", I, [I, I, I, D, MATVEC_FN, P, P, P, ctypes.POINTER(D), ctypes.POINTER(I), ctypes.POINTER(I), P, I,
                       ctypes.POINTER(I), P])

@@ -144,6 +144,16 @@ def set_wait_bound(seconds, force_timeout=False):
     _lib.check(_lib.lib.sie_set_wait_bound(float(seconds), 1 if force_timeout else 0))
 
 
+def mfma_f16_rate(iters=4000, reps=3):
+    """(TFLOP/s, MHz): the sustained FP16 matrix-core rate of the current
+    device under the split-FP16 GEMM's instruction mix and the shader clock
+    held meanwhile (register operands, no memory traffic)."""
+    tf, mhz = ctypes.c_double(), ctypes.c_double()
+    _lib.check(_lib.lib.sie_mfma_f16_rate(int(iters), int(reps), ctypes.byref(tf), ctypes.byref(mhz),
+                                          _stream()))
+    return tf.value, mhz.value
+
+
 class _DeviceVector:
     """Zero-copy view of n complex doubles at a device address (for torch.as_tensor)."""
 

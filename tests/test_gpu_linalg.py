@@ -447,3 +447,13 @@ def test_single_rhs_graph_replays(sm, op):
     M2 = A2 if op == "N" else A2.conj().T
     x = buf.cpu().numpy().astype(np.complex128)
     assert np.linalg.norm(M2 @ x - b) / (np.linalg.norm(M2) * np.linalg.norm(x)) < 5e-6
+
+
+def test_mfma_rate_calibration(sm):
+    """The sustained FP16 matrix-core rate lies below the published dense peak
+    (2.5 PFLOP/s at 2.4 GHz) and the clock below the peak clock."""
+    tf, mhz = sm.linalg.mfma_f16_rate(iters=2000, reps=2)
+    print(f"mfma f16 rate {tf:.0f} TFLOP/s at {mhz:.0f} MHz", flush=True)
+    assert 200.0 < tf < 2520.0
+    assert 500.0 < mhz < 2500.0
+    assert tf / 2516.6 <= mhz / 2400.0 * 1.02  # the rate follows the clock
