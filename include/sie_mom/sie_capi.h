@@ -288,6 +288,16 @@ int sie_set_wait_bound(double seconds, int force_timeout);
  * measurement for rooflines; no counterpart in the reference. Synchronises. */
 int sie_mfma_f16_rate(int iters, int reps, double* tflops, double* clock_mhz, void* stream);
 
+/* Launch form of the LU panel kernels, whose workgroups wait for one another
+ * and so must all be resident at once: 0 (default) plain dispatches, which
+ * the dispatcher makes resident on an otherwise idle device (at most one
+ * workgroup per CU); 1 cooperative launches, where the runtime guarantees it
+ * (~21 us more per panel). With plain dispatches, kernels of other streams
+ * running during a factorisation can keep a panel's workgroups out; its
+ * bounded waits then give up and sie_lu_factor returns SIE_ERR_DEVICE. No
+ * counterpart in the reference (cuSOLVER's panel). */
+int sie_set_panel_launch(int cooperative);
+
 /* Euclidean norm of n complex doubles (deterministic two-stage reduction,
  * synchronises the stream; returns -1 on error). */
 double sie_norm2(long long n, const double* x_dev, void* stream);

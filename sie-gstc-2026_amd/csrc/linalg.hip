@@ -1130,8 +1130,8 @@ __device__ __forceinline__ void read_rows_tagged(const unsigned long long* __res
 // their reads are agent-scope (device-coherent) accesses, so no L2 writeback
 // or invalidate is needed per column. Slots are per column and tags per
 // launch, so no stale word is taken for a current one. Three workgroup
-// barriers per column. Requires all G workgroups co-resident (cooperative
-// launch).
+// barriers per column. Requires all G workgroups co-resident (the launch
+// forms: g_cooperative_panels); the waits are bounded either way.
 template <int W, int BS>
 __global__ __launch_bounds__(BS) void panel_getf2_keyed_kernel(int M, int w, cx<float>* __restrict__ P,
                                                                long long lda, int* __restrict__ ipiv, int ipiv_off,
@@ -2188,6 +2188,15 @@ static WaitCtl& wait_ctl() {
     static WaitCtl c;
     return c;
 }
+// Launch form of the panel kernels (sie_set_panel_launch): plain dispatches by
+// default. A panel's workgroups wait for one another, so they must all be
+// resident at once; a plain dispatch of at most one workgroup per CU on a
+// stream whose previous kernel has finished gets that from the dispatcher
+// unless kernels of other streams hold the CUs, and then the bounded waits
+// give up (SIE_ERR_DEVICE). The cooperative form has the runtime guarantee it,
+// at ~21 us more per launch (178 against 157 us per 72k-row panel and two
+// small kernels, measured), 47 ms per factorisation at N = 72k.
+static bool g_cooperative_panels = false;
 static unsigned int* g_wait_word = nullptr;
 static unsigned int* wait_word() {
     if (!g_wait_word) {
@@ -2482,14 +2491,16 @@ static void panel(int M, int w, cx<T>* P, long long lda, int* ipiv, int ipiv_off
         WaitArg wa = wait_arg();
         void* args[] = {&M, &w, &P, &lda, &ipiv, &ipiv_off, &kcur, &knext, &nkeys, &pubw, &droww, &tag_base,
                         &d_info, &info_base, &wa};
-        SIE_HIP(hipLaunchCooperativeKernel(kern, dim3(G), dim3(kPanelBS), args, 0, s));
+        if (g_cooperative_panels) SIE_HIP(hipLaunchCooperativeKernel(kern, dim3(G), dim3(kPanelBS), args, 0, s));
+        else SIE_HIP(hipLaunchKernel(kern, dim3(G), dim3(kPanelBS), args, 0, s));
     } else {
         unsigned int base = st.counter_base;
         st.counter_base += (unsigned)(G * w);  // one arrival per workgroup and column
         WaitArg wa = wait_arg();
         void* args[] = {&M, &w, &P, &lda, &ipiv, &ipiv_off, &cand, &pub, &drow, &counter, &base, &d_info,
                         &info_base, &wa};
-        SIE_HIP(hipLaunchCooperativeKernel(kern, dim3(G), dim3(kPanelBS), args, 0, s));
+        if (g_cooperative_panels) SIE_HIP(hipLaunchCooperativeKernel(kern, dim3(G), dim3(kPanelBS), args, 0, s));
+        else SIE_HIP(hipLaunchKernel(kern, dim3(G), dim3(kPanelBS), args, 0, s));
     }
     prof().end(PROF_PANEL, s, 8.0 * M * w * w / 2, 2.0 * sizeof(cx<T>) * M * w);
 }
@@ -3254,6 +3265,12 @@ int sie_set_wait_bound(double seconds, int force_timeout) {
     require(seconds > 0 && seconds <= 3600, "wait bound must be in (0, 3600] seconds");
     wait_ctl().seconds = seconds;
     wait_ctl().force = force_timeout ? 1 : 0;
+    GUARD_END
+}
+
+int sie_set_panel_launch(int cooperative) {
+    GUARD_BEGIN
+    g_cooperative_panels = cooperative != 0;
     GUARD_END
 }
 

@@ -99,6 +99,7 @@ _sig("sie_convert", I, [I, I, LL, LL, P, LL, P, LL, D, P, P])
 _sig("sie_ir_solve", I, [I, I, I, P, LL, P, LL, P, P, LL, P, LL, D, D, I, P, P, P, P])
 _sig("sie_fgmres_ir_solve", I, [I, I, I, P, LL, P, LL, P, P, LL, P, LL, D, D, I, I, P, P, P, P])
 _sig("sie_set_wait_bound", I, [D, I])
+_sig("sie_set_panel_launch", I, [I])
 _sig("sie_release", I, [])
 _sig("sie_norm2", D, [LL, P, P])
 _sig("sie_mfma_f16_rate", I, [I, I, ctypes.POINTER(D), ctypes.POINTER(D), P])

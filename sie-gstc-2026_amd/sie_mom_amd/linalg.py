@@ -154,6 +154,13 @@ def mfma_f16_rate(iters=4000, reps=3):
     return tf.value, mhz.value
 
 
+def set_panel_launch(cooperative):
+    """Launch form of the LU panel kernels: plain dispatches (default) or
+    cooperative launches (the runtime guarantees that every workgroup of a
+    panel is resident; ~21 us more per panel)."""
+    _lib.check(_lib.lib.sie_set_panel_launch(1 if cooperative else 0))
+
+
 class _DeviceVector:
     """Zero-copy view of n complex doubles at a device address (for torch.as_tensor)."""
 

@@ -457,3 +457,27 @@ def test_mfma_rate_calibration(sm):
     assert 200.0 < tf < 2520.0
     assert 500.0 < mhz < 2500.0
     assert tf / 2516.6 <= mhz / 2400.0 * 1.02  # the rate follows the clock
+
+
+@pytest.mark.parametrize("prec,variant", [(64, "4m"), (32, "f16x3")])
+def test_panel_launch_forms(sm, prec, variant):
+    """Plain and cooperative launches of the panel kernels give bitwise the
+    same factors and pivots (n = 5000: panels of 5000 down to 8 rows)."""
+    n = 5000
+    rng = np.random.default_rng(77)
+    A = crandn(rng, n, n)
+    dt = torch.complex128 if prec == 64 else torch.complex64
+    out = []
+    try:
+        for coop in (False, True):
+            sm.linalg.set_panel_launch(coop)
+            print(f"panel launch cooperative={coop} prec={prec}", flush=True)
+            Ad = to_dev(A, dt)
+            lu = sm.linalg.DenseLuSolver(variant=variant)
+            lu.factorize(Ad)
+            out.append((Ad.cpu().numpy(), lu.pivots.cpu().numpy()))
+    finally:
+        sm.linalg.set_panel_launch(False)
+    assert np.array_equal(out[0][1], out[1][1])
+    fl = np.float64 if prec == 64 else np.float32
+    assert np.array_equal(np.ascontiguousarray(out[0][0]).view(fl), np.ascontiguousarray(out[1][0]).view(fl))
