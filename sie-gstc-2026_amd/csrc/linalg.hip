@@ -2836,6 +2836,13 @@ struct SweepGraphCache {
     long long inv_lda = 0;
     unsigned long long inv_gen = 0;
     const void* inv_w = nullptr;
+    // the row permutation held in the workspace
+    const void* perm_LU = nullptr;
+    const int* perm_ipiv = nullptr;
+    int perm_n = 0;
+    unsigned long long perm_gen = 0;
+    const void* perm_w = nullptr;
+    int perm_tsize = 0;
 };
 // Never destroyed (see ws()): graph executables and the capture stream are
 // released with the process.
@@ -2859,21 +2866,37 @@ void getrs_vec(int op, int n, const cx<TF>* LU, long long lda, const int* ipiv, 
     cx<T>* y = (cx<T>*)(w + 4 * invsz);
     cx<T>* t = y + n;
     int* perm = (int*)(t + n);
-    // the row permutation of P^T, composed on the host from the swap sequence
-    std::vector<int> hp(n), hpiv(n);
-    SIE_HIP(hipMemcpyAsync(hpiv.data(), ipiv, sizeof(int) * n, hipMemcpyDeviceToHost, s));
-    SIE_HIP(hipStreamSynchronize(s));
-    for (int i = 0; i < n; ++i) hp[i] = i;
-    for (int k = 0; k < n; ++k) {
-        require(hpiv[k] >= k && hpiv[k] < n, "pivot index outside the trailing rows (factors not from sie_lu_factor?)");
-        std::swap(hp[k], hp[hpiv[k]]);
-    }
-    SIE_HIP(hipMemcpyAsync(perm, hp.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
     SweepGraphCache& cache = sweep_graphs();
-    // inverses of the diagonal blocks: reused while the factors are those of
-    // the last factorisation of this array
     const auto it = factor_generation().find(LU);
     const unsigned long long gen = it == factor_generation().end() ? 0 : it->second;
+    // the row permutation of P^T, composed on the host from the swap sequence;
+    // reused while the factors and pivots are those of the last factorisation
+    // of this array (a copy of the pivots to the host and a pass over them per
+    // solve otherwise)
+    // (the permutation's place in the workspace depends on n and on the
+    // vector type, so both are part of the key)
+    if (!(gen != 0 && cache.perm_LU == LU && cache.perm_ipiv == ipiv && cache.perm_n == n && cache.perm_gen == gen &&
+          cache.perm_w == w && cache.perm_tsize == tkey)) {
+        std::vector<int> hp(n), hpiv(n);
+        SIE_HIP(hipMemcpyAsync(hpiv.data(), ipiv, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+        SIE_HIP(hipStreamSynchronize(s));
+        for (int i = 0; i < n; ++i) hp[i] = i;
+        for (int k = 0; k < n; ++k) {
+            require(hpiv[k] >= k && hpiv[k] < n,
+                    "pivot index outside the trailing rows (factors not from sie_lu_factor?)");
+            std::swap(hp[k], hp[hpiv[k]]);
+        }
+        SIE_HIP(hipMemcpyAsync(perm, hp.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+        SIE_HIP(hipStreamSynchronize(s));  // hp goes out of scope
+        cache.perm_LU = LU;
+        cache.perm_ipiv = ipiv;
+        cache.perm_n = n;
+        cache.perm_gen = gen;
+        cache.perm_w = w;
+        cache.perm_tsize = tkey;
+    }
+    // inverses of the diagonal blocks: reused while the factors are those of
+    // the last factorisation of this array
     if (!(gen != 0 && cache.inv_LU == LU && cache.inv_n == n && cache.inv_lda == lda &&
           cache.inv_tsize == tkey && cache.inv_gen == gen && cache.inv_w == w)) {
         diag_inv_kernel<TF, T, B><<<nb, B, 0, s>>>(n, LU, lda, 0, invL);
@@ -3252,6 +3275,9 @@ int sie_release(void) {
     c.inv_LU = nullptr;
     c.inv_w = nullptr;
     c.inv_gen = 0;
+    c.perm_LU = nullptr;
+    c.perm_w = nullptr;
+    c.perm_gen = 0;
     for (int i = 0; i < 8; ++i) ws(i).buf.release();
     if (g_wait_word) SIE_HIP(hipFree(g_wait_word));
     g_wait_word = nullptr;
