@@ -2512,7 +2512,11 @@ constexpr int kNB = 256;
 // factors its block by blocks of the next level and updates the rest of its
 // block once per sub-block with K = that width. Measured at N = 72k (FP32
 // split-FP16 updates): one level of 4096 columns over the NB = 256 steps
-// factors in 3.69 s, 1024 in 3.84 s, 8192 in 3.90 s and 4096 over 1024 in 3.80 s.
+// factors in 3.69 s, 1024 in 3.84 s, 8192 in 3.90 s and 4096 over 1024 in 3.80 s
+// (an earlier state of the kernels); with the present ones, on one device in
+// turn, 4096 alone 2.90 s against 4096 over 2048 2.92 s, and on other devices
+// 4096 over 1024 2.92 s, 8192 over 2048 2.89 s, 4096 over 2048 over 1024 2.94 s
+// (one level 2.86-2.91 s there).
 constexpr int kLevels[] = {4096};
 constexpr int kNumLevels = 1;
 
