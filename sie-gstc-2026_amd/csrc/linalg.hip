@@ -1072,30 +1072,11 @@ __device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
     return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
 }
 
-// One wave reads a tagged row of W <= 32 entries (2W words, one per lane)
-// into LDS, re-reading until every word carries the tag (bounded: the words
-// were stored before the key that led here was, so they arrive; a row still
-// incomplete at the bound is reported through *err).
-__device__ __forceinline__ void read_row_tagged(const unsigned long long* __restrict__ src, int W, unsigned tag,
-                                                float* __restrict__ out, unsigned int* err) {
-    const int lane = threadIdx.x & 63;
-    unsigned long long v = 0;
-    bool complete = false;
-    for (unsigned it = 0; it < (1u << 20); ++it) {
-        v = lane < 2 * W ? __hip_atomic_load(src + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                         : (unsigned long long)tag << 32;
-        if (__all((unsigned)(v >> 32) == tag)) {
-            complete = true;
-            break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-    }
-    if (!complete && lane == 0) flag_wait_error(err, kWaitRow);
-    if (lane < 2 * W) out[lane] = __uint_as_float((unsigned)v);
-}
-
-// The same for two rows at once (2W words each, two per lane), both carrying
-// the same tag: one round trip for the pivot row and row j.
+// Wave 0 reads two tagged rows of W <= 32 entries (2W words each, two per
+// lane: one round trip for the pivot row and row j), both carrying the same
+// tag, into LDS, re-reading until every word carries the tag (bounded: the
+// words were stored before the key that led here was, so they arrive; rows
+// still incomplete at the bound are reported through *err).
 __device__ __forceinline__ void read_rows_tagged(const unsigned long long* __restrict__ src1,
                                                  const unsigned long long* __restrict__ src2, int W, unsigned tag,
                                                  float* __restrict__ out1, float* __restrict__ out2,
