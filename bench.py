@@ -34,8 +34,18 @@ sys.path.insert(0, os.path.join(ROOT, "sie-gstc-2026_amd"))
 os.environ.setdefault("HIP_FORCE_DEV_KERNARG", "1")
 
 METRIC = "assemble+solve+adjoint-grad s/iter @N≈72k, FP32-MFMA+IR vs FP64, err≤1e-8"
-# refinement stop of both solves: normwise backward error |r| / (|Z|_F |x| + |b|)
-IR_TOL = 1e-15
+# refinement target of both solves: normwise backward error |r| / (|Z|_F |x| + |b|)
+# of 1e-17, the level of the plain FP64 LU solution's own backward error on
+# this operator (7e-18 to 2.5e-16 on the trajectory's iterates, reported as
+# fp64_backward_error): the mixed and the
+# FP64 solutions then differ by about cond(Z) (be_mixed + be_fp64), the order
+# of the FP64 solution's own error, also at the near-resonant iterates of the
+# trajectory where cond(Z) reaches ~1e8 (a stop at 1e-15 left x 8e-8 and the
+# gradient 2e-7 from FP64 there). A solve that stalls at the FP64 residual
+# floor above the target ends there (FGMRES-IR's stagnation exit) and is
+# accepted up to IR_ACCEPT.
+IR_TOL = 1e-17
+IR_ACCEPT = 1e-15
 
 WORKLOADS = {
     # name: (mesh kind, size parameter, quadrature rule, parameters per face)
@@ -160,7 +170,7 @@ def copy_calibration():
     print(json.dumps({"copy_bytes_read": x.numel() * 4, "copy_bytes_written": y.numel() * 4}))
 
 
-def lu_gemms(n=72048):
+def lu_gemms(n=72048, variant="f16x3"):
     """The split-FP16 products of the FP32 LU at N = n, on their own, in the
     factorisation's order and shapes: per 4096-column level (width w, first
     column c), after NB step s the in-level update C (M x N) -= L (M x 256)
@@ -185,7 +195,7 @@ def lu_gemms(n=72048):
         A = Lb[: M * K].view(K, M).T
         B = Ub[: N * K].view(N, K).T
         C = Cbuf[: M * N].view(N, M).T
-        sm.linalg.gemm(A, B, C, alpha=-1.0, beta=1.0, variant="f16x3")
+        sm.linalg.gemm(A, B, C, alpha=-1.0, beta=1.0, variant=variant)
         launches += 1
 
     for c in range(0, n, 4096):
@@ -197,7 +207,7 @@ def lu_gemms(n=72048):
             upd(w - 256 * (st + 1), right, 256)
         upd(right, right, 4096)
     torch.cuda.synchronize()
-    print(json.dumps({"lu_gemm_launches": launches, "n": n}))
+    print(json.dumps({"lu_gemm_launches": launches, "n": n, "variant": variant}))
 
 
 def _pmc_mean_kib(csv_path, counter, name_part):
@@ -391,7 +401,7 @@ def pmchwt_bench(args):
 
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "--lu-gemms":
-        return lu_gemms()
+        return lu_gemms(variant=sys.argv[2] if len(sys.argv) > 2 else "f16x3")
     if len(sys.argv) > 1 and sys.argv[1] == "--copy-calibration":
         return copy_calibration()
     if len(sys.argv) > 1 and sys.argv[1] == "--summarize-pmc":
@@ -410,7 +420,7 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=2,
                     help="timed FP64 iterations (after one untimed FP64 iteration on the same iterate)")
     ap.add_argument("--ir-tol", type=float, default=IR_TOL,
-                    help="refinement stop: normwise backward error |r| / (|Z|_F |x| + |b|)")
+                    help="refinement target: normwise backward error |r| / (|Z|_F |x| + |b|)")
     ap.add_argument("--no-matvec", action="store_true", help="skip the matrix-free product measurement")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -444,7 +454,8 @@ def main():
     rng = np.random.default_rng(20260101)
     params = torch.from_numpy(rng.uniform(-2, 2, size=(mesh.nf, kind)) / k0).cuda()
     problem = ad.GstcProblem(mesh, o, waves, kind, mode="mixed", ir_tol=args.ir_tol, ir_maxit=40,
-                             variant=args.variant, ir_method=args.ir_method, ir_stop="backward")
+                             variant=args.variant, ir_method=args.ir_method, ir_stop="backward",
+                             ir_accept=IR_ACCEPT)
     lib = _lib.lib
     # load every kernel once on a tiny problem (code objects, workspaces)
     tiny = sm.Mesh.icosphere(1.0, 1)
@@ -510,7 +521,7 @@ def main():
     ir_log.clear()
     fw = [e[0] for e in timed_log]
     aw = [e[1] for e in timed_log]
-    ir_stats = {"stop": "normwise backward error |r|/(|Z|_F |x|+|b|)", "tol": args.ir_tol,
+    ir_stats = {"stop": "normwise backward error |r|/(|Z|_F |x|+|b|)", "tol": args.ir_tol, "accept": IR_ACCEPT,
                 "forward_steps": fw, "adjoint_steps": aw,
                 "forward_steps_mean": sum(fw) / len(fw), "forward_steps_max": max(fw),
                 "adjoint_steps_mean": sum(aw) / len(aw), "adjoint_steps_max": max(aw),
@@ -526,20 +537,32 @@ def main():
         x_mixed = st.xs.clone()
         g_mixed = g.clone()
         J_mixed = float(J)
-        del st
-        torch.cuda.empty_cache()
         p64 = ad.GstcProblem(mesh, o, waves, kind, mode="fp64")
         # one untimed FP64 iteration of the last iterate (allocation, first
         # launches; its solution is the reference for the errors), then the
         # same iteration timed fp64_steps times like the mixed one (whole
-        # steps, no per-launch events)
+        # steps, no per-launch events). The mixed state (its FP64 operator) is
+        # kept through the untimed iteration for the backward errors of both
+        # solutions against the same operator (208 GB at N = 72k).
         st64 = p64.solve(p_last)
         J64, F64 = p64.objective(st64)
         g64 = p64.gradient(st64, F64)
         ex = float(torch.linalg.norm(x_mixed - st64.xs) / torch.linalg.norm(st64.xs))
         eg = float(torch.linalg.norm(g_mixed - g64) / torch.linalg.norm(g64))
         eJ = abs(J_mixed - float(J64)) / abs(float(J64))
-        del st64
+        bvec = sm.ops.rhs_plane_waves(mesh, o, waves)[:, 0].contiguous()
+        bn = float(torch.linalg.norm(bvec))
+
+        def backward_error(x):
+            r = sm.linalg.zgemv(st.Z, x, bvec.clone(), alpha=-1.0, beta=1.0)
+            return float(torch.linalg.norm(r)) / (st.znorm * float(torch.linalg.norm(x)) + bn)
+
+        be64, bem = backward_error(st64.xs), backward_error(x_mixed)
+        log(f"backward error on the last iterate's operator: fp64 LU solution {be64:.2e}, mixed {bem:.2e}")
+        # freed into torch's cache, not released: the timed FP64 iterations then
+        # take their operator from the cache, as the mixed ones do (a release
+        # here put a fresh 83 GB allocation into the first timed FP64 assembly)
+        del st64, st, bvec
         p64.timings = {}
         t64 = None
         if args.fp64_steps > 0:
@@ -555,6 +578,7 @@ def main():
         log(f"mixed vs fp64: rel err x {ex:.2e}, grad {eg:.2e}, J {eJ:.2e}")
         out_extra = {"fp64_s_per_iter": t64, "fp64_steps": args.fp64_steps,
                      "speedup_vs_fp64": (t64 / s_per_iter) if t64 else None, "rel_err_x_vs_fp64": ex,
+                     "fp64_backward_error": be64, "mixed_backward_error": bem,
                      "rel_err_grad_vs_fp64": eg, "rel_err_J_vs_fp64": eJ,
                      "fp64_phases_s": {k: round(v / max(args.fp64_steps, 1), 4) for k, v in p64.timings.items()}}
         torch.cuda.empty_cache()

@@ -251,7 +251,9 @@ int sie_ir_solve(int op, int n, int nrhs, const double* Z_dev, long long ldz, co
  * (GMRES-IR): z_k = LU32^{-1} v_k with FP64 arithmetic on the FP32 factors,
  * w = op(Z) z_k and its CGS2 orthogonalisation in FP64, x += [z_0..z_{k-1}] y;
  * each cycle ends with the true FP64 residual. Same stopping rule and outputs
- * as sie_ir_solve; iterations[c] counts inner steps (one preconditioner solve
+ * as sie_ir_solve, and one more: a restart cycle that does not halve the true
+ * residual ends the solve (the residual's FP64 floor), with the backward error
+ * reached reported; iterations[c] counts inner steps (one preconditioner solve
  * and one FP64 GEMV each). No counterpart in the reference. Synchronises the
  * stream every step. */
 int sie_fgmres_ir_solve(int op, int n, int nrhs, const double* Z_dev, long long ldz,

@@ -198,7 +198,11 @@ int gmres_run(int n, int m, int maxnit, double tol, sie_matvec_fn matvec, void* 
 // gives the relative-residual test |r| <= tol |b|. Inside a cycle |x| is
 // estimated by beta |z_0| (the preconditioned first direction) in the first
 // cycle and by the last true |x| afterwards; each cycle's end tests the true
-// residual and |x|.
+// residual and |x|. A restart cycle that does not halve the true residual
+// ends the solve: the residual has reached the floor of its FP64 evaluation
+// (the Arnoldi estimate inside a cycle keeps falling below it, so further
+// cycles would only run to maxit); the caller compares the backward error
+// reached with its own acceptance bound.
 int fgmres_ir_run(int op, int n, const double* Z, long long ldz, const float* LU, long long ldl, const int* ipiv,
                   const double* b, double* x, double tol, double znorm, int m, int maxit, double* out_rel,
                   double* out_be, hipStream_t s) {
@@ -218,7 +222,8 @@ int fgmres_ir_run(int op, int n, const double* Z, long long ldz, const float* LU
     double beta = bn, xn = 0.0;
     bool have_xn = false;
     auto bound = [&](double xnorm) { return tol * (znorm * xnorm + bn); };
-    int total = 0;
+    int total = 0, cycles = 0;
+    double prev_beta = beta;
     while (beta > bound(xn) && total < maxit) {
         gm_copy_scaled_kernel<<<gb, 256, 0, s>>>(n, r.p, beta, V.p);
         SIE_HIP(hipGetLastError());
@@ -293,6 +298,8 @@ int fgmres_ir_run(int op, int n, const double* Z, long long ldz, const float* LU
             xn = sie_norm2(n, x, s);
             have_xn = true;
         }
+        if (++cycles >= 2 && beta > 0.5 * prev_beta) break;  // stagnation at the FP64 residual floor
+        prev_beta = beta;
     }
     if (out_rel) *out_rel = beta / bn;
     if (out_be) *out_be = beta / (znorm * xn + bn);

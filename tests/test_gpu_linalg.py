@@ -55,6 +55,31 @@ def test_gemm_alpha_beta(sm, prec, variant, alpha, beta, m, n, k):
     assert err < (1e-15 if prec == 64 else 4e-7), err
 
 
+@pytest.mark.parametrize("alpha,beta", [(-1.0, 1.0), (0.5, 2.0), (-3.0, 0.25), (2.0, 0.0)])
+@pytest.mark.parametrize("m,n,k", [(600, 1100, 512), (300, 2049, 1000), (1100, 1030, 4096)])
+def test_gemm_wide(sm, alpha, beta, m, n, k):
+    # the wide split-FP16 products (k >= 512, n > 1024: the LU's level updates,
+    # 256 x 128 tiles, three LDS stages); ragged tiles and k not a multiple of 16
+    variant = "f16x3"
+    g = torch.Generator(device="cuda").manual_seed(m + n + k)
+    A = torch.randn(k, m, dtype=torch.complex128, device="cuda", generator=g).T
+    B = torch.randn(n, k, dtype=torch.complex128, device="cuda", generator=g).T
+    C = torch.randn(n, m, dtype=torch.complex128, device="cuda", generator=g).T
+    Cd = C.to(torch.complex64)
+    print(f"gemm wide {variant} alpha={alpha} beta={beta} m={m} n={n} k={k}", flush=True)
+    sm.linalg.gemm(A.to(torch.complex64), B.to(torch.complex64), Cd, alpha=alpha, beta=beta, variant=variant)
+    A32, B32, C32 = (x.to(torch.complex64).to(torch.complex128) for x in (A, B, C))
+    ref = beta * C32 + alpha * (A32 @ B32)
+    scale = abs(alpha) * A32.abs().max() * B32.abs().max() * k + abs(beta) * C32.abs().max()
+    err = ((Cd.to(torch.complex128) - ref).abs().max() / scale).item()
+    # normwise (Frobenius) error relative to |A| |B|
+    fro = (torch.linalg.norm(Cd.to(torch.complex128) - ref) /
+           (abs(alpha) * torch.linalg.norm(A32) * torch.linalg.norm(B32) + abs(beta) * torch.linalg.norm(C32))).item()
+    print(f"  max err {err:.2e}, normwise {fro:.2e}", flush=True)
+    assert err < 2e-7, err
+    assert fro < 1e-7, fro
+
+
 def well_conditioned(rng, n):
     return crandn(rng, n, n) + 2 * np.sqrt(n) * np.eye(n)
 
@@ -198,6 +223,27 @@ def test_fgmres_ir_solve(sm, op):
         assert np.linalg.norm(M @ x - b) / np.linalg.norm(b) < 1e-13
 
 
+def test_fgmres_ir_floor(sm):
+    """A backward-error target far below FP64 (1e-30): FGMRES-IR refines to the
+    residual's FP64 floor and stops at the first restart cycle that does not
+    halve the residual, long before maxit, with that floor's backward error."""
+    rng = np.random.default_rng(12)
+    n = 500
+    A = crandn(rng, n, n) + 2 * np.sqrt(n) * np.eye(n)
+    b = crandn(rng, n, 1)
+    Zd = to_dev(A)
+    LU32, zf = sm.linalg.to_precision(Zd, 32, fro=True)
+    lu = sm.linalg.DenseLuSolver(variant="f16x3")
+    lu.factorize(LU32)
+    X, rr, its, be = sm.linalg.ir_solve(Zd, LU32, lu.pivots, to_dev(b), tol=1e-30, maxit=400, method="fgmres",
+                                        restart=20, znorm=zf)
+    print(f"floor: {its[0]} steps, backward error {be[0]:.2e}", flush=True)
+    assert its[0] < 100
+    assert be[0] < 1e-16
+    xr = np.linalg.solve(A, b)
+    assert np.linalg.norm(X.cpu().numpy() - xr) / np.linalg.norm(xr) < 1e-14
+
+
 def test_fgmres_ir_restarts(sm):
     """A restart length of 2 forces several cycles (each ending with the true
     residual) and still converges."""
@@ -243,8 +289,9 @@ def test_ir_solve_many_rhs(sm, variant):
 def test_backward_error_stop(sm, method, cond):
     """Ill-conditioned systems with a right-hand side outside the dominant
     range, so |x| is ~cond |b| and no solve reaches |r| / |b| = 1e-12 (FP64
-    rounding alone leaves ~u |A| |x| / |b|): that test runs to maxit and
-    reports it. With the Frobenius norm from the conversion the refinement
+    rounding alone leaves ~u |A| |x| / |b|): that test reports it unreached,
+    classic IR after maxit sweeps, FGMRES-IR at the first restart cycle that
+    does not halve the residual. With the Frobenius norm from the conversion the refinement
     stops at the normwise backward error |r| / (|A|_F |x| + |b|) <= 1e-15,
     which FP64 LU itself attains, and the result matches the host FP64 solve
     to the conditioning bound. (A host simulation with FP32 LU factors takes
@@ -262,7 +309,7 @@ def test_backward_error_stop(sm, method, cond):
     lu = sm.linalg.DenseLuSolver(variant="f16x3")
     lu.factorize(LU32)
     _, rr0, it0, _ = sm.linalg.ir_solve(Zd, LU32, lu.pivots, to_dev(b), tol=1e-12, maxit=40, method=method)
-    assert rr0[0] > 1e-12 and it0[0] == 40
+    assert rr0[0] > 1e-12 and (it0[0] == 40 if method == "ir" else it0[0] < 40)
     X, rr, its, be = sm.linalg.ir_solve(Zd, LU32, lu.pivots, to_dev(b), tol=1e-15, maxit=40, method=method,
                                         znorm=fro)
     x = X.cpu().numpy()
