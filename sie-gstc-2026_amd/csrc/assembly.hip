@@ -38,6 +38,7 @@
 #include "trig.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 
 namespace sie {
@@ -678,12 +679,14 @@ __global__ __launch_bounds__(256) void gather_kernel(
     int nf, int ne, int n0, int nc, const int* __restrict__ elist, int nlist,
     const int* __restrict__ ef, const int* __restrict__ el, const cplx* __restrict__ W,
     const cplx* __restrict__ M /* nf x 144, GSTC only */, Epi ep, cplx* __restrict__ Z,
-    long long ldz, int col_stride, int overwrite) {
+    long long ldz, int col_stride, int overwrite, int wave_pass = -1) {
     // col_stride 0: columns of the full matrix (edge id + block E); > 0: a slab
     // whose block c holds the chunk's edges in list order at c * col_stride.
     // overwrite: the only chunk (every face pair present), so every entry of
     // the lane's blocks is stored, zeros included, and Z need not be cleared
     // first; otherwise the chunk's contributions are added.
+    // wave_pass (T-PMCHWT, NK = 1 blocks): the blocks of wavenumber k_(pass+1)
+    // only, so that Z is the sum of one pass per region's wavenumber.
     constexpr int NS = NK * 18;
     const int em = blockIdx.x * blockDim.x + threadIdx.x;
     const int li = uniform(blockIdx.y);
@@ -766,6 +769,15 @@ __global__ __launch_bounds__(256) void gather_kernel(
             }
     } else if (KIND == K_PEC) {
         zput(em, col(0), cmul(ep.c[0], NK == 2 ? acc[2] : acc[0]));
+    } else if (KIND == K_PMCHWT && NK == 1 && wave_pass >= 0) {
+        // one region's terms: jw mu_p TL_p, eta0 TK_p, jw eps_p TL_p (eta0^2 scaled)
+        const cplx TL = acc[0], TK = acc[1];
+        const cplx zjj = cmul(ep.c[wave_pass], TL), zmm = cmul(ep.c[2 + wave_pass], TL);
+        const double et = ep.eta, et2 = ep.jm;
+        zput(em, col(0), zjj);
+        zput(em, col(1), {TK.re * et, TK.im * et});
+        zput(em + E, col(0), {-TK.re * et, -TK.im * et});
+        zput(em + E, col(1), {zmm.re * et2, zmm.im * et2});
     } else if (KIND == K_PMCHWT) {
         const cplx TL1 = acc[0], TK1 = acc[1];
         const cplx TL2 = NK == 2 ? acc[2] : acc[0], TK2 = NK == 2 ? acc[3] : acc[1];
@@ -1091,9 +1103,11 @@ struct Assembler {
     }
 
     // Runs steps 1-3 for source faces [n0, n0+nc) into W.
+    // count_geom = false: a second pass over the same pairs for another
+    // wavenumber (its geometric work is not algorithmic work again)
     template <int NK>
     void chunk_blocks(int n0, int nc, double k1, double k2, float thr, DevBuf<double>& area, hipStream_t s,
-                      bool sym = false) {
+                      bool sym = false, bool count_geom = true) {
         const int nf = mesh->nf;
         const int Q = quad.n;
         // near list
@@ -1120,7 +1134,7 @@ struct Assembler {
         // flops of the moment-factored point-pair update (kFarPairFlops per pair and wavenumber),
         // over the pairs integrated (m > n only when symmetric)
         const double pairs = sym ? 0.5 * (double)nf * (nf - 1) : (double)nf * nc;
-        prof().end(PROF_FAR_PAIRS, s, pairs * Q * Q * (kFarGeomFlops + NK * kFarPairFlops),
+        prof().end(PROF_FAR_PAIRS, s, pairs * Q * Q * ((count_geom ? kFarGeomFlops : 0) + NK * kFarPairFlops),
                    (double)nf * nc * NK * 18 * 16.0);
         SIE_HIP(hipGetLastError());
         SIE_HIP(hipStreamSynchronize(s));
@@ -1144,12 +1158,24 @@ struct Assembler {
     }
 
     // One chunk of every source face, integrated with the pair symmetry, when
-    // its scratch (F^2 NK 18 complex) stays under kSymBytes and fits in free
-    // device memory with a margin; otherwise scratch_bytes-sized chunks.
+    // its scratch (F^2 NK 18 complex) stays under the cap and fits in free
+    // device memory with a margin; otherwise scratch_bytes-sized chunks. The
+    // cap is kSymBytes for the scratch of every wavenumber together and
+    // kSymWaveBytes for one wavenumber at a time (T-PMCHWT with two regions
+    // whose joint scratch does not fit: one symmetric pass per wavenumber does
+    // 53 of the 95 flops per point pair of one non-symmetric pass with both);
+    // SIE_SYM_SCRATCH_BYTES, when set, replaces both (tests reach the
+    // per-wavenumber form at small meshes).
     static constexpr size_t kSymBytes = size_t(64) << 30;
-    bool symmetric_fits(int NK) const {
+    static constexpr size_t kSymWaveBytes = size_t(160) << 30;
+    static size_t sym_cap(bool per_wave) {
+        const char* e = std::getenv("SIE_SYM_SCRATCH_BYTES");
+        if (e != nullptr && std::atoll(e) > 0) return (size_t)std::atoll(e);
+        return per_wave ? kSymWaveBytes : kSymBytes;
+    }
+    bool symmetric_fits(int NK, bool per_wave = false) const {
         const size_t need = (size_t)mesh->nf * mesh->nf * NK * 18 * sizeof(cplx);
-        if (need > kSymBytes) return false;
+        if (need > sym_cap(per_wave)) return false;
         size_t fr = 0, tot = 0;
         if (hipMemGetInfo(&fr, &tot) != hipSuccess) return false;
         return need + (size_t(8) << 30) <= fr + W.n * sizeof(cplx);
@@ -1196,6 +1222,32 @@ static void assemble_generic(sie_mesh* h, const sie_calderon_params* p, const Ep
     // assembly into Z integrates each far pair once (both orientations);
     // products keep the chunked form so that the slab stays a few columns wide
     const bool sym = x == nullptr && A.symmetric_fits(NK);
+    if constexpr (NK == 2 && KIND == K_PMCHWT) {
+        if (x == nullptr && !sym && A.symmetric_fits(1, true)) {
+            // one symmetric pass per region's wavenumber: the first stores every
+            // entry of Z, the second adds its terms
+            const size_t w1 = (size_t)m.nf * 18 * m.nf;
+            if (A.W.n < w1) A.W.alloc(w1);
+            std::vector<int> all(m.ne);
+            for (int e = 0; e < m.ne; ++e) all[e] = e;
+            if (A.elist.n < all.size()) A.elist.alloc(m.ne);
+            SIE_HIP(hipMemcpyAsync(A.elist.p, all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice, s));
+            dim3 grid((m.ne + 255) / 256, (unsigned)m.ne);
+            for (int pass = 0; pass < 2; ++pass) {
+                const double k = pass == 0 ? (double)p->k1 : (double)p->k2;
+                A.chunk_blocks<1>(0, m.nf, k, k, p->threshold, h->area, s, true, pass == 0);
+                prof().begin(PROF_GATHER, s);
+                gather_kernel<1, KIND><<<grid, 256, 0, s>>>(m.nf, m.ne, 0, m.nf, A.elist.p, m.ne, m.d_ef.p, m.d_el.p,
+                                                            A.W.p, Mblk, ep, Z, ldz, 0, pass == 0 ? 1 : 0, pass);
+                prof().end(PROF_GATHER, s, 0.0,
+                           (double)m.nf * m.nf * 18 * 16.0 + (double)dim * 2 * m.ne * 16.0 * (pass == 0 ? 1 : 2));
+                SIE_HIP(hipGetLastError());
+                SIE_HIP(hipStreamSynchronize(s));
+            }
+            if (w1 * sizeof(cplx) > A.scratch_bytes) A.W.release();
+            return;
+        }
+    }
     const int nc_max = sym ? m.nf : A.chunk_faces(NK);
     const size_t wsz = (size_t)nc_max * NK * 18 * m.nf;
     if (A.W.n < wsz) A.W.alloc(wsz);
