@@ -116,8 +116,11 @@ __global__ void near_list_kernel(int nf, int n0, int nc, const double* __restric
 // swapped pair (test n, source m) with the transposed blocks, which are equal:
 // TL(n, m)[i][j] = TL(m, n)[j][i] because the same rule integrates both faces,
 // and TK likewise since swapping the faces flips both rho_j x rho_i and r - r'.
+// One wavenumber: at most 168 registers, three waves per SIMD (the compiler
+// keeps a few of the per-test-point accumulators outside the point loop in
+// scratch: 2.6% less time for the far pairs at 72k than at two waves).
 template <int NK, bool SYM>
-__global__ __launch_bounds__(256) void far_pairs_kernel(
+__global__ __launch_bounds__(256, NK == 1 ? 3 : 1) void far_pairs_kernel(
     int nf, int n0, int nc, int Q, const double* __restrict__ c, const double* __restrict__ o,
     const double* __restrict__ len, const double* __restrict__ bq, const float* __restrict__ qxy,
     const float* __restrict__ qw, const double* __restrict__ sctab, double k1, double k2, float thr,
@@ -1161,11 +1164,9 @@ struct Assembler {
     // its scratch (F^2 NK 18 complex) stays under the cap and fits in free
     // device memory with a margin; otherwise scratch_bytes-sized chunks. The
     // cap is kSymBytes for the scratch of every wavenumber together and
-    // kSymWaveBytes for one wavenumber at a time (T-PMCHWT with two regions
-    // whose joint scratch does not fit: one symmetric pass per wavenumber does
-    // 53 of the 95 flops per point pair of one non-symmetric pass with both);
-    // SIE_SYM_SCRATCH_BYTES, when set, replaces both (tests reach the
-    // per-wavenumber form at small meshes).
+    // kSymWaveBytes for one wavenumber at a time (T-PMCHWT with two regions:
+    // one symmetric pass per wavenumber); SIE_SYM_SCRATCH_BYTES, when set,
+    // replaces both (tests reach the chunked form at small meshes).
     static constexpr size_t kSymBytes = size_t(64) << 30;
     static constexpr size_t kSymWaveBytes = size_t(160) << 30;
     static size_t sym_cap(bool per_wave) {
@@ -1223,9 +1224,14 @@ static void assemble_generic(sie_mesh* h, const sie_calderon_params* p, const Ep
     // products keep the chunked form so that the slab stays a few columns wide
     const bool sym = x == nullptr && A.symmetric_fits(NK);
     if constexpr (NK == 2 && KIND == K_PMCHWT) {
-        if (x == nullptr && !sym && A.symmetric_fits(1, true)) {
-            // one symmetric pass per region's wavenumber: the first stores every
-            // entry of Z, the second adds its terms
+        if (x == nullptr && A.symmetric_fits(1, true)) {
+            // one symmetric pass per region's wavenumber, the first storing every
+            // entry of Z, the second adding its terms: half the scratch of both
+            // wavenumbers together (at N = 61,440 the joint one does not fit
+            // beside Z) and the kernel for one wavenumber holds 168 registers
+            // (three waves per SIMD) where the joint one needs 360 (one wave):
+            // at N = 15,360 the far pairs take 211 against 250 ms, at 61,440
+            // 3.1 s against 5.6 s for the non-symmetric chunks of both
             const size_t w1 = (size_t)m.nf * 18 * m.nf;
             if (A.W.n < w1) A.W.alloc(w1);
             std::vector<int> all(m.ne);
