@@ -372,8 +372,10 @@ __device__ Wil wilton(V3 r, const V3 t[3], V3 n) {
     return o;
 }
 
+// One wavenumber: held to two waves per SIMD (256 registers, a few spilled;
+// it needs 300 otherwise, one wave): 22 against 29 ms at 72k.
 template <int NK>
-__global__ __launch_bounds__(128) void near_pairs_kernel(
+__global__ __launch_bounds__(128, NK == 1 ? 2 : 1) void near_pairs_kernel(
     int npairs, const int2* __restrict__ list, int nf, int n0, int Q, const double* __restrict__ V,
     const int* __restrict__ F, const double* __restrict__ len, const double* __restrict__ area,
     const float* __restrict__ qxy, const float* __restrict__ qw, double k1, double k2,
