@@ -1074,7 +1074,7 @@ struct Assembler {
     QuadDev quad;
     DevBuf<double> bq;
     int bq_rule = 0;
-    DevBuf<double> sctab;  // e^{j 2 pi i / 256}: (cos, sin) pairs for sincos_tab
+    DevBuf<double> sctab;  // e^{j 2 pi i / 2048}: (cos, sin) pairs for sincos_tab
     DevBuf<cplx> W;
     DevBuf<int2> near;
     DevBuf<unsigned int> count;
