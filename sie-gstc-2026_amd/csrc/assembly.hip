@@ -47,10 +47,10 @@ namespace {
 
 constexpr double kPi = 3.14159265358979323846;
 // FP64 operations per point pair in far_pairs_kernel, counted on the source:
-// geometry (R, |R|^2, |R|, kR, weight scale) and, per wavenumber, G, g and the
-// 8 complex moment accumulations. sqrt/rsqrt and sincos are counted as one
-// operation each (their instruction sequences are not).
-constexpr double kFarGeomFlops = 3 + 5 + 1 + 2;
+// geometry (|R|^2 = |P|^2 + |b|^2 - 2 P.b, |R|, kR, weight scale) and, per
+// wavenumber, G, g and the 8 complex moment accumulations. sqrt/rsqrt and
+// sincos are counted as one operation each (their instruction sequences are not).
+constexpr double kFarGeomFlops = 7 + 1 + 2;
 constexpr double kFarPairFlops = 1 + 1 + 2 + 6 + 4 + 28;
 constexpr double kInv4Pi = 1.0 / (4.0 * kPi);
 
@@ -82,16 +82,20 @@ __device__ __forceinline__ bool is_near(const double* __restrict__ c, int nf, in
 }
 
 // ------------------------------------------------------------------ step 0
-// Source points relative to their centroid, b_q = r_q - c (mapTriQuadPoint).
+// Source points relative to their centroid, b_q = r_q - c (mapTriQuadPoint),
+// and |b_q|^2: bq[4 t .. 4 t + 3] = (b_x, b_y, b_z, |b|^2).
 __global__ void source_points_kernel(int nf, int Q, const double* __restrict__ o,
                                      const float* __restrict__ qxy, double* __restrict__ bq) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nf * Q) return;
     const int f = t / Q, q = t - f * Q;
     const double xi = qxy[2 * q], eta = qxy[2 * q + 1], ze = 1.0 - xi - eta;
-    for (int x = 0; x < 3; ++x)
-        bq[3 * (size_t)t + x] = o[(0 + x) * (size_t)nf + f] * ze + o[(3 + x) * (size_t)nf + f] * xi +
-                                o[(6 + x) * (size_t)nf + f] * eta;
+    double b[3];
+    for (int x = 0; x < 3; ++x) {
+        b[x] = o[(0 + x) * (size_t)nf + f] * ze + o[(3 + x) * (size_t)nf + f] * xi + o[(6 + x) * (size_t)nf + f] * eta;
+        bq[4 * (size_t)t + x] = b[x];
+    }
+    bq[4 * (size_t)t + 3] = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
 }
 
 // ------------------------------------------------------------------ step 1
@@ -163,7 +167,7 @@ __global__ __launch_bounds__(256, NK == 1 ? 3 : 1) void far_pairs_kernel(
 #pragma unroll
         for (int x = 0; x < 3; ++x) { Sa[w][x] = Sb[w][x] = Ua[w][x] = Ub[w][x] = Uab[w][x] = {0, 0}; }
     }
-    const double* bqn = bq + (size_t)n * Q * 3;
+    const double* bqn = bq + (size_t)n * Q * 4;
 
     for (int p = 0; p < Q; ++p) {
         const double xi = qxy[2 * p], eta = qxy[2 * p + 1], wp = qw[p];
@@ -174,6 +178,16 @@ __global__ __launch_bounds__(256, NK == 1 ? 3 : 1) void far_pairs_kernel(
             ap[x] = om[0][x] * ze + om[1][x] * xi + om[2][x] * eta;
             P[x] = dlt[x] + ap[x];
         }
+        // |R|^2 = |P - b_q|^2 = (|P|^2 + |b_q|^2) - 2 P.b_q: four operations per
+        // point pair instead of six (the rounding error, ~4 eps (|P|^2 + |b|^2),
+        // is relative to |R|^2 within 20 eps on far pairs; near pairs take
+        // the same R in the pass that removes the static terms below)
+        const double m2P[3] = {-2.0 * P[0], -2.0 * P[1], -2.0 * P[2]};
+        const double PP = P[0] * P[0] + P[1] * P[1] + P[2] * P[2];
+        auto rsq = [&](int q) {
+            const double* b = bqn + 4 * q;
+            return fma(m2P[0], b[0], fma(m2P[1], b[1], fma(m2P[2], b[2], PP + b[3])));
+        };
         cplx T0[NK], Tb[NK][3], V0[NK], Vb[NK][3];
 #pragma unroll
         for (int w = 0; w < NK; ++w) {
@@ -182,9 +196,8 @@ __global__ __launch_bounds__(256, NK == 1 ? 3 : 1) void far_pairs_kernel(
             for (int x = 0; x < 3; ++x) { Tb[w][x] = {0, 0}; Vb[w][x] = {0, 0}; }
         }
         for (int q = 0; q < Q; ++q) {
-            const double bx = bqn[3 * q], by = bqn[3 * q + 1], bz = bqn[3 * q + 2];
-            const double Rx = P[0] - bx, Ry = P[1] - by, Rz = P[2] - bz;
-            const double R2 = Rx * Rx + Ry * Ry + Rz * Rz;
+            const double bx = bqn[4 * q], by = bqn[4 * q + 1], bz = bqn[4 * q + 2];
+            const double R2 = rsq(q);
             // R2 > 0 on far pairs; lanes of other pairs compute values that are not stored
             const double iR = rsqrt_pos(R2);
             const double Rm = R2 * iR;
@@ -213,10 +226,8 @@ __global__ __launch_bounds__(256, NK == 1 ? 3 : 1) void far_pairs_kernel(
             // w_q (1/R^3 + k^2/(2R))/(4 pi) in g, taken out of the real moments
             // (a separate pass, so that the loop above stays that of far pairs)
             for (int q = 0; q < Q; ++q) {
-                const double bx = bqn[3 * q], by = bqn[3 * q + 1], bz = bqn[3 * q + 2];
-                const double Rx = P[0] - bx, Ry = P[1] - by, Rz = P[2] - bz;
-                const double R2 = Rx * Rx + Ry * Ry + Rz * Rz;
-                const double iR = rsqrt_pos(R2);
+                const double bx = bqn[4 * q], by = bqn[4 * q + 1], bz = bqn[4 * q + 2];
+                const double iR = rsqrt_pos(rsq(q));
                 const double gs = rg * swq[q] * iR;
 #pragma unroll
                 for (int w = 0; w < NK; ++w) {
@@ -1097,7 +1108,7 @@ struct Assembler {
             SIE_HIP(hipMemcpy(sctab.p, h, sizeof(h), hipMemcpyHostToDevice));
         }
         if (bq_rule != rule) {
-            bq.alloc((size_t)mesh->nf * quad.n * 3);
+            bq.alloc((size_t)mesh->nf * quad.n * 4);
             const int tot = mesh->nf * quad.n;
             source_points_kernel<<<(tot + 255) / 256, 256, 0, s>>>(mesh->nf, quad.n, mesh->d_o.p, quad.xy.p, bq.p);
             SIE_HIP(hipGetLastError());
