@@ -1329,6 +1329,114 @@ __global__ void laswp_kernel(cx<T>* __restrict__ A, long long lda, int rows, int
     }
 }
 
+// The interchanges of rows kb..ke-1 (ipiv, applied in order, p > k) as one
+// permutation, for at most L interchanges. Every workgroup composes it in LDS
+// (step j interchanges rows k = kb + j and p_j):
+//   last(j): the last earlier step whose pivot row is kb + j;
+//   prev(j): the last earlier step with the same pivot row as step j;
+//   A(j) = the original row found at kb + j just before step j
+//        = A(last(j)) if last(j) exists, else kb + j (pointer jumping);
+// row kb + j then ends with A(prev(j)) if prev(j) exists, else p_j (step j
+// interchanges) or with A(j) (no interchange at j), and a pivot row p >= ke
+// ends with A(j) of the last step that chose it. Then each column's moved
+// entries are all loaded before any is stored. cols: columns per workgroup.
+template <typename T, int L>
+__global__ __launch_bounds__(256) void laswp_perm_kernel(cx<T>* __restrict__ A, long long lda, int rows, int c0,
+                                                         int nc, int c1, int nc1, const int* __restrict__ ipiv,
+                                                         int kb, int ke, int cols) {
+    constexpr int MV = 2 * L;                   // moved rows at most
+    constexpr int PER = (MV + 255) / 256;       // moves per thread
+    constexpr int CB = 16 / (PER * sizeof(T) / 4) > 0 ? 16 / (PER * sizeof(T) / 4) : 1;  // columns per batch
+    __shared__ int sp[L], last[L], prv[L], par[L], fin[L];
+    __shared__ int mpos[MV], msrc[MV];
+    __shared__ int nmv;
+    const int tid = threadIdx.x, nL = ke - kb;
+    if (tid == 0) nmv = 0;
+    for (int j = tid; j < L; j += 256) {
+        int p = -1;
+        if (j < nL) {
+            p = ipiv[kb + j];
+            if (!(p > kb + j && p < rows)) p = -1;
+        }
+        sp[j] = p;
+        last[j] = -1;
+        fin[j] = 1;
+    }
+    __syncthreads();
+    for (int j = tid; j < nL; j += 256) {
+        const int p = sp[j];
+        int pr = -1;
+        if (p >= 0) {
+            if (p < ke) atomicMax(&last[p - kb], j);
+            for (int i = 0; i < j; ++i)
+                if (sp[i] == p) pr = i;
+        }
+        prv[j] = pr;
+    }
+    __syncthreads();
+    for (int j = tid; j < nL; j += 256) {
+        par[j] = last[j] >= 0 ? last[j] : j;
+        if (prv[j] >= 0) fin[prv[j]] = 0;  // a later step chose the same pivot row
+    }
+    __syncthreads();
+    for (int span = 1; span < L; span *= 2) {
+        int np[(L + 255) / 256];
+#pragma unroll
+        for (int u = 0; u < (L + 255) / 256; ++u) {
+            const int j = tid + 256 * u;
+            np[u] = j < nL ? par[par[j]] : 0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < (L + 255) / 256; ++u) {
+            const int j = tid + 256 * u;
+            if (j < nL) par[j] = np[u];
+        }
+        __syncthreads();
+    }
+    for (int j = tid; j < nL; j += 256) {
+        const int p = sp[j];
+        const int src = p < 0 ? kb + par[j] : prv[j] >= 0 ? kb + par[prv[j]] : p;
+        if (src != kb + j) {
+            const int t = atomicAdd(&nmv, 1);
+            mpos[t] = kb + j;
+            msrc[t] = src;
+        }
+        if (p >= ke && fin[j]) {
+            const int t = atomicAdd(&nmv, 1);
+            mpos[t] = p;
+            msrc[t] = kb + par[j];
+        }
+    }
+    __syncthreads();
+    const int nm = nmv;
+    const int cbeg = blockIdx.x * cols, cend = min(cbeg + cols, nc + nc1);
+    for (int cb = cbeg; cb < cend; cb += CB) {
+        cx<T> v[CB][PER];
+#pragma unroll
+        for (int b = 0; b < CB; ++b) {
+            const int c = cb + b;
+            cx<T>* col = A + (long long)(c < nc ? c0 + c : c1 + (c - nc)) * lda;
+#pragma unroll
+            for (int u = 0; u < PER; ++u) {
+                const int i = tid + 256 * u;
+                v[b][u] = (c < cend && i < nm) ? col[msrc[i]] : cx<T>{0, 0};
+            }
+        }
+        __syncthreads();  // every entry of these columns read before any is written
+#pragma unroll
+        for (int b = 0; b < CB; ++b) {
+            const int c = cb + b;
+            cx<T>* col = A + (long long)(c < nc ? c0 + c : c1 + (c - nc)) * lda;
+#pragma unroll
+            for (int u = 0; u < PER; ++u) {
+                const int i = tid + 256 * u;
+                if (c < cend && i < nm) col[mpos[i]] = v[b][u];
+            }
+        }
+    }
+}
+
 // ============================================== small triangular solves
 // In-place solve of a w x w (w <= WM) triangular system on ncols right-hand columns:
 //   uplo 0: unit lower L (forward), 1: non-unit upper U (backward),
@@ -2344,7 +2452,22 @@ static void laswp2(cx<T>* A, long long lda, int rows, int c0, int nc, int c1, in
     nc = std::max(nc, 0);
     nc1 = std::max(nc1, 0);
     if (nc + nc1 <= 0 || ke <= kb) return;
-    laswp_kernel<T><<<(nc + nc1 + 127) / 128, 128, 0, s>>>(A, lda, rows, c0, nc, c1, nc1, ipiv, kb, ke, 0, 0);
+    // up to 256 interchanges (a panel's, an NB block's): composed per
+    // workgroup and moved with the loads in flight together; longer
+    // sequences (a level's 4096, on the wide trailing columns) one column per
+    // lane, in order (bound by the scattered pivot rows' traffic)
+    const int tot = nc + nc1, len = ke - kb;
+    if (len <= 32) {
+        constexpr int cols = 16;
+        laswp_perm_kernel<T, 32><<<(tot + cols - 1) / cols, 256, 0, s>>>(A, lda, rows, c0, nc, c1, nc1, ipiv, kb, ke,
+                                                                         cols);
+    } else if (len <= 256) {
+        constexpr int cols = 16;
+        laswp_perm_kernel<T, 256><<<(tot + cols - 1) / cols, 256, 0, s>>>(A, lda, rows, c0, nc, c1, nc1, ipiv, kb,
+                                                                          ke, cols);
+    } else {
+        laswp_kernel<T><<<(tot + 127) / 128, 128, 0, s>>>(A, lda, rows, c0, nc, c1, nc1, ipiv, kb, ke, 0, 0);
+    }
     SIE_HIP(hipGetLastError());
 }
 
