@@ -154,6 +154,37 @@ def test_lu_deferred_swaps(sm, prec, variant, monkeypatch):
     assert be < (1e-14 if prec == 64 else 5e-6), be
 
 
+@pytest.mark.parametrize("prec,variant", [(64, "4m"), (32, "f16x3")])
+@pytest.mark.parametrize("pattern", ["repeat", "chain"])
+def test_lu_pivot_chains(sm, prec, variant, pattern):
+    # A = rows of an upper-triangular U with a dominant diagonal, placed so that
+    # partial pivoting picks, at every step k, U's row k from a known position:
+    # "repeat" - always the last row (the same pivot row for all n - 1 steps,
+    # outside every block of interchanges); "chain" - row k + 1 (each
+    # interchange moves the row the previous one brought in). The multipliers
+    # are zero, so the factors are U exactly and only the row interchanges can
+    # be wrong; n = 5000 crosses panels, NB blocks and a 4096-column level.
+    n = 5000
+    dt = torch.complex128 if prec == 64 else torch.complex64
+    g = torch.Generator(device="cuda").manual_seed(5000)
+    U = torch.triu(0.01 * torch.randn(n, n, dtype=dt, device="cuda", generator=g), 1)
+    U += torch.diag(4.0 + torch.rand(n, dtype=torch.float64, device="cuda", generator=g).to(dt))
+    rows = torch.arange(n, device="cuda")
+    if pattern == "repeat":
+        src = torch.cat([rows[1:], rows[:1]])         # position j holds U row j + 1, the last U row 0
+        expect = np.full(n, n - 1)
+    else:
+        src = torch.cat([rows[-1:], rows[:-1]])       # position 0 holds U row n - 1, position j + 1 U row j
+        expect = np.minimum(np.arange(n) + 1, n - 1)
+    A = U[src].T.contiguous().T                       # column-major
+    print(f"lu pivot chains prec={prec} {variant} {pattern} n={n}", flush=True)
+    lu = sm.linalg.DenseLuSolver(variant=variant)
+    lu.factorize(A)
+    piv = lu.pivots.cpu().numpy()
+    assert np.array_equal(piv, expect), np.flatnonzero(piv != expect)[:10]
+    assert torch.equal(A, U)
+
+
 def test_singular_reported(sm):
     n = 64
     rng = np.random.default_rng(1)
