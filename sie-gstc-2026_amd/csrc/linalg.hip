@@ -3104,21 +3104,44 @@ void getrs(int op, int n, int nrhs, const cx<T>* LU, long long lda, const int* i
         return;
     }
     if (op == 0) {
-        laswp<T>(B, ldb, n, 0, nrhs, ipiv, 0, n, 0, false, s);
-        // L y = b (unit lower), forward
-        for (int k = 0; k < n; k += BS) {
-            const int w = std::min(BS, n - k);
-            trsm_small<T>(w, LU + (long long)k * lda + k, lda, B + k, ldb, nrhs, 0, s);
-            const int below = n - k - w;
-            if (below > 0)
-                gemm<T>(below, nrhs, w, -1.0, LU + (long long)k * lda + k + w, lda, B + k, ldb, 1.0, B + k + w, ldb,
-                        s, 0);
+        // the interchanges, composed per NB of them
+        for (int k = 0; k < n; k += kNB) laswp2<T>(B, ldb, n, 0, nrhs, 0, 0, ipiv, k, std::min(k + kNB, n), s);
+        // NB-row blocks: inside a block, BS-row triangles and their K = BS
+        // updates of the block's other rows; the rows beyond the block take one
+        // K = NB product (FP32: the split-FP16 GEMM, FP32-accurate)
+        auto block_update = [&](int M, const cx<T>* Lk, cx<T>* Bk, cx<T>* Ck, int K, float a_bound) {
+            if (M <= 0) return;
+            if constexpr (std::is_same<T, float>::value) {
+                gemm_f16x3(M, nrhs, K, -1.0, Lk, lda, Bk, ldb, 1.0, Ck, ldb, s, a_bound);
+            } else {
+                (void)a_bound;
+                gemm<T>(M, nrhs, K, -1.0, Lk, lda, Bk, ldb, 1.0, Ck, ldb, s, 0);
+            }
+        };
+        // L y = b (unit lower, multipliers bounded by sqrt 2), forward
+        for (int kb = 0; kb < n; kb += kNB) {
+            const int nb = std::min(kNB, n - kb);
+            for (int k = kb; k < kb + nb; k += BS) {
+                const int w = std::min(BS, kb + nb - k);
+                trsm_small<T>(w, LU + (long long)k * lda + k, lda, B + k, ldb, nrhs, 0, s);
+                const int below = kb + nb - k - w;
+                if (below > 0)
+                    gemm<T>(below, nrhs, w, -1.0, LU + (long long)k * lda + k + w, lda, B + k, ldb, 1.0, B + k + w,
+                            ldb, s, 0);
+            }
+            block_update(n - kb - nb, LU + (long long)kb * lda + kb + nb, B + kb, B + kb + nb, nb, 1.5f);
         }
         // U x = y, backward
-        for (int k = ((n - 1) / BS) * BS; k >= 0; k -= BS) {
-            const int w = std::min(BS, n - k);
-            trsm_small<T>(w, LU + (long long)k * lda + k, lda, B + k, ldb, nrhs, 1, s);
-            if (k > 0) gemm<T>(k, nrhs, w, -1.0, LU + (long long)k * lda, lda, B + k, ldb, 1.0, B, ldb, s, 0);
+        for (int kb = ((n - 1) / kNB) * kNB; kb >= 0; kb -= kNB) {
+            const int nb = std::min(kNB, n - kb);
+            for (int k = kb + ((nb - 1) / BS) * BS; k >= kb; k -= BS) {
+                const int w = std::min(BS, kb + nb - k);
+                trsm_small<T>(w, LU + (long long)k * lda + k, lda, B + k, ldb, nrhs, 1, s);
+                if (k > kb)
+                    gemm<T>(k - kb, nrhs, w, -1.0, LU + (long long)k * lda + kb, lda, B + k, ldb, 1.0, B + kb, ldb,
+                            s, 0);
+            }
+            block_update(kb, LU + (long long)kb * lda, B + kb, B, nb, 0.f);
         }
     } else {
         // A^H = U^H L^H P^T: U^H z = b (forward), L^H w = z (backward), x = P w
