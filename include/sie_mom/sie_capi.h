@@ -197,8 +197,8 @@ int sie_lu_factor(int precision, int n, void* A_dev, long long lda, int* ipiv_de
  * op = 1 (A^T) is solved as A^H conj(X) = conj(B). With nrhs <= 4 (blocked
  * sweeps that wait on published blocks) the call synchronises the stream and
  * returns SIE_ERR_DEVICE if such a wait gave up. With more columns and op = 0
- * the rows beyond each 256-row block are updated by one product (precision
- * 32: the FP32-accurate split-FP16 product of sie_gemm variant 2). */
+ * the rows beyond each 256-row block are updated by one product in the
+ * factors' precision (each column accurate relative to its own scale). */
 int sie_lu_solve(int precision, int op, int n, int nrhs, const void* LU_dev, long long lda,
                  const int* ipiv_dev, void* B_dev, long long ldb, void* stream);
 

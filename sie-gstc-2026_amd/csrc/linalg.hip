@@ -3083,18 +3083,23 @@ __global__ void conj_kernel(int n, int nrhs, cx<T>* __restrict__ B, long long ld
 
 template <typename T>
 void getrs(int op, int n, int nrhs, const cx<T>* LU, long long lda, const int* ipiv, cx<T>* B, long long ldb,
-           hipStream_t s);
+           hipStream_t s, bool split_f16 = false);
 
 // Solve op(A) X = B with the factors of getrf (op 0 = N, 1 = T, 2 = C).
-// A^T X = B is solved as A^H conj(X) = conj(B).
+// A^T X = B is solved as A^H conj(X) = conj(B). split_f16 (FP32 factors, op
+// N, more than 4 columns): the products beyond each 256-row block on the
+// split-FP16 matrix cores, whose one power-of-two scale per operand makes
+// them accurate relative to the largest entry of all columns; for callers
+// whose columns are of one scale (the blocked refinement's normalised
+// corrections). Otherwise those products are FP32.
 template <typename T>
 void getrs(int op, int n, int nrhs, const cx<T>* LU, long long lda, const int* ipiv, cx<T>* B, long long ldb,
-           hipStream_t s) {
+           hipStream_t s, bool split_f16) {
     constexpr int BS = 32;
     if (op == 1) {
         const unsigned g = (unsigned)std::min<long long>(((long long)n * nrhs + 255) / 256, 1 << 16);
         conj_kernel<T><<<g, 256, 0, s>>>(n, nrhs, B, ldb);
-        getrs<T>(2, n, nrhs, LU, lda, ipiv, B, ldb, s);
+        getrs<T>(2, n, nrhs, LU, lda, ipiv, B, ldb, s, false);
         conj_kernel<T><<<g, 256, 0, s>>>(n, nrhs, B, ldb);
         SIE_HIP(hipGetLastError());
         return;
@@ -3108,15 +3113,17 @@ void getrs(int op, int n, int nrhs, const cx<T>* LU, long long lda, const int* i
         for (int k = 0; k < n; k += kNB) laswp2<T>(B, ldb, n, 0, nrhs, 0, 0, ipiv, k, std::min(k + kNB, n), s);
         // NB-row blocks: inside a block, BS-row triangles and their K = BS
         // updates of the block's other rows; the rows beyond the block take one
-        // K = NB product (FP32: the split-FP16 GEMM, FP32-accurate)
+        // K = NB product
         auto block_update = [&](int M, const cx<T>* Lk, cx<T>* Bk, cx<T>* Ck, int K, float a_bound) {
             if (M <= 0) return;
             if constexpr (std::is_same<T, float>::value) {
-                gemm_f16x3(M, nrhs, K, -1.0, Lk, lda, Bk, ldb, 1.0, Ck, ldb, s, a_bound);
-            } else {
-                (void)a_bound;
-                gemm<T>(M, nrhs, K, -1.0, Lk, lda, Bk, ldb, 1.0, Ck, ldb, s, 0);
+                if (split_f16) {
+                    gemm_f16x3(M, nrhs, K, -1.0, Lk, lda, Bk, ldb, 1.0, Ck, ldb, s, a_bound);
+                    return;
+                }
             }
+            (void)a_bound;
+            gemm<T>(M, nrhs, K, -1.0, Lk, lda, Bk, ldb, 1.0, Ck, ldb, s, 0);
         };
         // L y = b (unit lower, multipliers bounded by sqrt 2), forward
         for (int kb = 0; kb < n; kb += kNB) {
@@ -3283,7 +3290,7 @@ static void ir_blocked(int n, int nrhs, const c64* Z, long long ldz, const c32* 
         SIE_HIP(hipMemcpyAsync(gain.p, hg.data(), sizeof(double) * nrhs, hipMemcpyHostToDevice, s));
         scale_cols_f32_kernel<<<g, 256, 0, s>>>(n, nrhs, R.p, n, sc.p, D.p, n);
         SIE_HIP(hipGetLastError());
-        getrs<float>(0, n, nrhs, LU, ldl, ipiv, D.p, n, s);
+        getrs<float>(0, n, nrhs, LU, ldl, ipiv, D.p, n, s, true);  // columns normalised above
         axpy_cols_kernel<<<g, 256, 0, s>>>(n, nrhs, D.p, n, gain.p, X, ldx);
         SIE_HIP(hipGetLastError());
         // R = B - Z X
