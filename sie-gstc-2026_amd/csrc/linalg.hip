@@ -1340,10 +1340,12 @@ __global__ void laswp_kernel(cx<T>* __restrict__ A, long long lda, int rows, int
 // interchanges) or with A(j) (no interchange at j), and a pivot row p >= ke
 // ends with A(j) of the last step that chose it. Then each column's moved
 // entries are all loaded before any is stored. cols: columns per workgroup.
+// reverse: the interchanges in the opposite order (the inverse permutation:
+// every move taken backwards).
 template <typename T, int L>
 __global__ __launch_bounds__(256) void laswp_perm_kernel(cx<T>* __restrict__ A, long long lda, int rows, int c0,
                                                          int nc, int c1, int nc1, const int* __restrict__ ipiv,
-                                                         int kb, int ke, int cols) {
+                                                         int kb, int ke, int cols, int reverse = 0) {
     constexpr int MV = 2 * L;                   // moved rows at most
     constexpr int PER = (MV + 255) / 256;       // moves per thread
     constexpr int CB = 16 / (PER * sizeof(T) / 4) > 0 ? 16 / (PER * sizeof(T) / 4) : 1;  // columns per batch
@@ -1410,6 +1412,8 @@ __global__ __launch_bounds__(256) void laswp_perm_kernel(cx<T>* __restrict__ A, 
     }
     __syncthreads();
     const int nm = nmv;
+    const int* from = reverse ? mpos : msrc;
+    const int* to = reverse ? msrc : mpos;
     const int cbeg = blockIdx.x * cols, cend = min(cbeg + cols, nc + nc1);
     for (int cb = cbeg; cb < cend; cb += CB) {
         cx<T> v[CB][PER];
@@ -1420,7 +1424,7 @@ __global__ __launch_bounds__(256) void laswp_perm_kernel(cx<T>* __restrict__ A, 
 #pragma unroll
             for (int u = 0; u < PER; ++u) {
                 const int i = tid + 256 * u;
-                v[b][u] = (c < cend && i < nm) ? col[msrc[i]] : cx<T>{0, 0};
+                v[b][u] = (c < cend && i < nm) ? col[from[i]] : cx<T>{0, 0};
             }
         }
         __syncthreads();  // every entry of these columns read before any is written
@@ -1431,7 +1435,7 @@ __global__ __launch_bounds__(256) void laswp_perm_kernel(cx<T>* __restrict__ A, 
 #pragma unroll
             for (int u = 0; u < PER; ++u) {
                 const int i = tid + 256 * u;
-                if (c < cend && i < nm) col[mpos[i]] = v[b][u];
+                if (c < cend && i < nm) col[to[i]] = v[b][u];
             }
         }
     }
@@ -3081,6 +3085,28 @@ __global__ void conj_kernel(int n, int nrhs, cx<T>* __restrict__ B, long long ld
     }
 }
 
+// D = S^H for an r x c block S (column-major, lds) into D (c x r, ldd):
+// 32 x 32 tiles through LDS, loads and stores coalesced.
+template <typename T>
+__global__ __launch_bounds__(256) void conj_transpose_kernel(int r, int c, const cx<T>* __restrict__ S, long long lds,
+                                                             cx<T>* __restrict__ D, long long ldd) {
+    __shared__ cx<T> t[32][33];
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8 threads
+    const long long r0 = (long long)blockIdx.x * 32, c0 = (long long)blockIdx.y * 32;
+    for (int k = ty; k < 32; k += 8) {
+        const long long i = r0 + tx, j = c0 + k;  // S(i, j)
+        if (i < r && j < c) t[k][tx] = S[j * lds + i];
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const long long a = c0 + tx, b = r0 + k;  // D(a, b) = conj(S(b, a))
+        if (a < c && b < r) {
+            const cx<T> v = t[tx][k];
+            D[b * ldd + a] = cx<T>{v.re, -v.im};
+        }
+    }
+}
+
 template <typename T>
 void getrs(int op, int n, int nrhs, const cx<T>* LU, long long lda, const int* ipiv, cx<T>* B, long long ldb,
            hipStream_t s, bool split_f16 = false);
@@ -3151,38 +3177,44 @@ void getrs(int op, int n, int nrhs, const cx<T>* LU, long long lda, const int* i
             block_update(kb, LU + (long long)kb * lda, B + kb, B, nb, 0.f);
         }
     } else {
-        // A^H = U^H L^H P^T: U^H z = b (forward), L^H w = z (backward), x = P w
-        cx<double>* part = (cx<double>*)ws(2).get((size_t)((n + 4095) / 4096) * BS * sizeof(cx<double>));
-        for (int k = 0; k < n; k += BS) {
-            const int w = std::min(BS, n - k);
-            // b_k -= (U^H)(k-block, 0:k) z(0:k) = conj(U(0:k, k-block))^T z
-            if (k > 0) {
-                for (int c = 0; c < nrhs; ++c) {
-                    const int chunk = 4096;
-                    const int nch = (k + chunk - 1) / chunk;
-                    gemv_c_partial<T, T><<<dim3((w + 3) / 4, nch), 256, 0, s>>>(k, w, chunk, LU + (long long)k * lda, lda,
-                                                                            B + (long long)c * ldb, part);
-                    gemv_finish<T><<<(w + 255) / 256, 256, 0, s>>>(w, nch, part, -1.0, 1.0, B + (long long)c * ldb + k);
-                }
+        // A^H = U^H L^H P^T: U^H z = b (forward), L^H w = z (backward), x = P w.
+        // Right-looking in NB-row blocks as for op N; the products take the
+        // needed rows of U (columns of L) conjugate-transposed into a
+        // temporary: C -= S^H X with S an r x c block of the factors.
+        cx<T>* tmp = (cx<T>*)ws(2).get((size_t)n * kNB * sizeof(cx<T>));
+        auto update_h = [&](int r, int c, const cx<T>* S, const cx<T>* X, cx<T>* C) {
+            if (r <= 0 || c <= 0) return;
+            conj_transpose_kernel<T><<<dim3((unsigned)((r + 31) / 32), (unsigned)((c + 31) / 32)), 256, 0, s>>>(
+                r, c, S, lda, tmp, c);
+            gemm<T>(c, nrhs, r, -1.0, tmp, c, X, ldb, 1.0, C, ldb, s, 0);
+        };
+        // U^H z = b: U^H is lower; the rows of U of a solved block reach the
+        // entries below it
+        for (int kb = 0; kb < n; kb += kNB) {
+            const int nb = std::min(kNB, n - kb);
+            for (int k = kb; k < kb + nb; k += BS) {
+                const int w = std::min(BS, kb + nb - k);
+                trsm_small<T>(w, LU + (long long)k * lda + k, lda, B + k, ldb, nrhs, 2, s);
+                update_h(w, kb + nb - k - w, LU + (long long)(k + w) * lda + k, B + k, B + k + w);
             }
-            trsm_small<T>(w, LU + (long long)k * lda + k, lda, B + k, ldb, nrhs, 2, s);
+            update_h(nb, n - kb - nb, LU + (long long)(kb + nb) * lda + kb, B + kb, B + kb + nb);
         }
-        for (int k = ((n - 1) / BS) * BS; k >= 0; k -= BS) {
-            const int w = std::min(BS, n - k);
-            const int below = n - k - w;
-            if (below > 0) {
-                // w_k -= (L^H)(k-block, below) w(below) = conj(L(below, k-block))^T w(below)
-                for (int c = 0; c < nrhs; ++c) {
-                    const int chunk = 4096;
-                    const int nch = (below + chunk - 1) / chunk;
-                    gemv_c_partial<T, T><<<dim3((w + 3) / 4, nch), 256, 0, s>>>(
-                        below, w, chunk, LU + (long long)k * lda + k + w, lda, B + (long long)c * ldb + k + w, part);
-                    gemv_finish<T><<<(w + 255) / 256, 256, 0, s>>>(w, nch, part, -1.0, 1.0, B + (long long)c * ldb + k);
-                }
+        // L^H w = z: L^H is unit upper; the columns of L of a solved block
+        // reach the entries above it
+        for (int kb = ((n - 1) / kNB) * kNB; kb >= 0; kb -= kNB) {
+            const int nb = std::min(kNB, n - kb);
+            for (int k = kb + ((nb - 1) / BS) * BS; k >= kb; k -= BS) {
+                const int w = std::min(BS, kb + nb - k);
+                trsm_small<T>(w, LU + (long long)k * lda + k, lda, B + k, ldb, nrhs, 3, s);
+                update_h(w, k - kb, LU + (long long)kb * lda + k, B + k, B + kb);
             }
-            trsm_small<T>(w, LU + (long long)k * lda + k, lda, B + k, ldb, nrhs, 3, s);
+            update_h(nb, kb, LU + kb, B + kb, B);
         }
-        laswp<T>(B, ldb, n, 0, nrhs, ipiv, 0, n, 0, true, s);
+        // x = P w: the interchanges backwards, NB of them at a time, last first
+        for (int k = ((n - 1) / kNB) * kNB; k >= 0; k -= kNB) {
+            const int ke = std::min(k + kNB, n);
+            laswp_perm_kernel<T, kNB><<<(nrhs + 15) / 16, 256, 0, s>>>(B, ldb, n, 0, nrhs, 0, 0, ipiv, k, ke, 16, 1);
+        }
     }
     SIE_HIP(hipGetLastError());
 }

@@ -441,22 +441,25 @@ def test_lu_solve_wide_rhs(sm, prec, op):
 
 
 @pytest.mark.parametrize("prec", [64, 32])
-def test_lu_solve_many_rhs_scaled(sm, prec):
-    # op N with more than 4 columns over several 256-row blocks (the block
-    # products), columns 20 orders of magnitude apart: each column's backward
-    # error is relative to its own scale
+@pytest.mark.parametrize("op", ["N", "C"])
+def test_lu_solve_many_rhs_scaled(sm, prec, op):
+    # more than 4 columns over several 256-row blocks (the block products; op C
+    # with the factors' rows and columns conjugate-transposed), columns 20
+    # orders of magnitude apart: each column's backward error is relative to
+    # its own scale
     rng = np.random.default_rng(23)
     n, nrhs = 1000, 12
     A = crandn(rng, n, n)
     b = crandn(rng, n, nrhs) * np.logspace(-10, 10, nrhs)[None, :]
     dt = torch.complex128 if prec == 64 else torch.complex64
-    print(f"lu many rhs scaled prec={prec}", flush=True)
+    print(f"lu many rhs scaled prec={prec} op={op}", flush=True)
     lu = sm.linalg.DenseLuSolver(variant="4m" if prec == 64 else "f16x3")
     lu.factorize(to_dev(A, dt))
     Bd = to_dev(b, dt)
-    lu.solve(Bd)
+    lu.solve(Bd, op=op)
+    M = A if op == "N" else A.conj().T
     X = Bd.cpu().numpy().astype(np.complex128)
-    res = np.linalg.norm(A @ X - b, axis=0) / (np.linalg.norm(A) * np.linalg.norm(X, axis=0))
+    res = np.linalg.norm(M @ X - b, axis=0) / (np.linalg.norm(M) * np.linalg.norm(X, axis=0))
     assert res.max() < (1e-14 if prec == 64 else 5e-6), res
 
 
