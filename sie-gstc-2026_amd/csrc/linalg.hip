@@ -1346,9 +1346,12 @@ template <typename T, int L>
 __global__ __launch_bounds__(256) void laswp_perm_kernel(cx<T>* __restrict__ A, long long lda, int rows, int c0,
                                                          int nc, int c1, int nc1, const int* __restrict__ ipiv,
                                                          int kb, int ke, int cols, int reverse = 0) {
-    constexpr int MV = 2 * L;                   // moved rows at most
-    constexpr int PER = (MV + 255) / 256;       // moves per thread
-    constexpr int CB = 16 / (PER * sizeof(T) / 4) > 0 ? 16 / (PER * sizeof(T) / 4) : 1;  // columns per batch
+    constexpr int MV = 2 * L;              // moved rows at most
+    constexpr int PER = (MV + 255) / 256;  // moves per thread
+    // columns per batch: a thread keeps CB x PER complex entries in flight,
+    // 32 registers (an FP32 entry takes 2, an FP64 entry 4)
+    constexpr int REGS = PER * (int)sizeof(T) / 2;  // registers per column
+    constexpr int CB = 32 / REGS > 0 ? 32 / REGS : 1;
     __shared__ int sp[L], last[L], prv[L], par[L], fin[L];
     __shared__ int mpos[MV], msrc[MV];
     __shared__ int nmv;
