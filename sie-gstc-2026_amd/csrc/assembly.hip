@@ -9,11 +9,12 @@
 //                          self pair in a moment-factored form (below), with
 //                          the regularised kernels on near pairs, writing the
 //                          raw TL/TK 3x3 blocks of each pair to the scratch W.
-//   3. near_pairs_kernel - one lane per near pair: the Wilton singular part
+//   3. self_pairs_kernel - one workgroup per self pair: its regularised
+//                          quadrature (dunavant_quadrature.cuh:207-324 with
+//                          is_singular) into its slots of W;
+//      near_pairs_kernel - one lane per near pair: the Wilton singular part
 //                          (dunavant_quadrature.cuh:769-875) added to the near
-//                          slots of W; self pairs get their regularised
-//                          quadrature here too (dunavant_quadrature.cuh:207-324
-//                          with is_singular).
+//                          slots of W.
 //   4. gather kernels    - one lane per (test edge, source edge): sum the four
 //                          face-pair contributions of the RWG pair in a fixed
 //                          order, apply the formulation's epilogue (and the
@@ -409,6 +410,87 @@ __device__ Wil wilton(V3 r, const V3 t[3], V3 n) {
 
 // One wavenumber: held to two waves per SIMD (256 registers, a few spilled;
 // it needs 300 otherwise, one wave): 22 against 29 ms at 72k.
+// Regular part of the self pairs (m, m) of the chunk's source faces: the
+// regularised double quadrature (integrandKernelsRegular) in the reference's
+// direct form, with its limit -jk/(4 pi) where the two points coincide; one
+// workgroup per face, its threads over the Q x Q point pairs, then a
+// fixed-order reduction (wave shuffles, then the four waves in order). It
+// stores TL (TK vanishes on a flat face); near_pairs_kernel then adds the
+// singular part. (One lane per self pair, as the reference loops, left 63 of
+// 64 lanes idle through Q^2 point pairs.)
+template <int NK>
+__global__ __launch_bounds__(256) void self_pairs_kernel(int nf, int n0, int Q, const double* __restrict__ V,
+                                                         const int* __restrict__ F, const double* __restrict__ len,
+                                                         const float* __restrict__ qxy, const float* __restrict__ qw,
+                                                         double k1, double k2, cplx* __restrict__ W) {
+    constexpr int NS = NK * 18, NV = NK * 18;
+    __shared__ double part[4][NV];
+    const int m = n0 + blockIdx.x, tid = threadIdx.x;
+    V3 tt[3], tf[3];
+    for (int a = 0; a < 3; ++a) {
+        const int v = F[3 * m + a];
+        tt[a] = {V[3 * v], V[3 * v + 1], V[3 * v + 2]};
+    }
+    for (int a = 0; a < 3; ++a) tf[a] = tt[(a + 2) % 3];
+    const double kk[2] = {k1, k2};
+    double acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+    for (int pq = tid; pq < Q * Q; pq += 256) {
+        const int p = pq / Q, q = pq - p * Q;
+        const double xp = qxy[2 * p], yp = qxy[2 * p + 1], wp = qw[p];
+        const double zp = 1.0 - xp - yp;
+        const V3 rp = {tt[0].x * zp + tt[1].x * xp + tt[2].x * yp, tt[0].y * zp + tt[1].y * xp + tt[2].y * yp,
+                       tt[0].z * zp + tt[1].z * xp + tt[2].z * yp};
+        const double xq = qxy[2 * q], yq = qxy[2 * q + 1], wq = qw[q];
+        const double zq = 1.0 - xq - yq;
+        const V3 rq = {tt[0].x * zq + tt[1].x * xq + tt[2].x * yq, tt[0].y * zq + tt[1].y * xq + tt[2].y * yq,
+                       tt[0].z * zq + tt[1].z * xq + tt[2].z * yq};
+        const V3 R = vsub(rp, rq);
+        const double Rm = sqrt(vdot(R, R));
+        const double cw = wp * wq;
+#pragma unroll
+        for (int w = 0; w < NK; ++w) {
+            cplx G;
+            if (Rm < 1e-9) {
+                G = {0.0, -kk[w] * kInv4Pi};
+            } else {
+                const double iR = 1.0 / Rm;
+                double sn, co;
+                sincos_moderate(kk[w] * Rm, &sn, &co);
+                G = {(co - 1.0) * kInv4Pi * iR, -sn * kInv4Pi * iR};
+            }
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const V3 rm = vsub(rp, tf[j]);
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    const V3 rn = vsub(rq, tf[i]);
+                    const double md = 0.25 * vdot(rm, rn) - 1.0 / (kk[w] * kk[w]);
+                    acc[w * 18 + (j * 3 + i) * 2] += G.re * md * cw;
+                    acc[w * 18 + (j * 3 + i) * 2 + 1] += G.im * md * cw;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+        for (int off = 32; off > 0; off >>= 1) acc[v] += __shfl_xor(acc[v], off);
+    if ((tid & 63) == 0)
+        for (int v = 0; v < NV; ++v) part[tid >> 6][v] = acc[v];
+    __syncthreads();
+    if (tid < NK * 9) {
+        const int w = tid / 9, ji = tid % 9, j = ji / 3, i = ji % 3;
+        const int v = w * 18 + ji * 2;
+        const double sc = len[j * (size_t)nf + m] * len[i * (size_t)nf + m];
+        const double re = ((part[0][v] + part[1][v]) + (part[2][v] + part[3][v])) * sc;
+        const double im = ((part[0][v + 1] + part[1][v + 1]) + (part[2][v + 1] + part[3][v + 1])) * sc;
+        cplx* Wm = W + (size_t)(m - n0) * NS * nf + m;
+        Wm[(size_t)((w * 2 + 0) * 9 + ji) * nf] = {re, im};
+        Wm[(size_t)((w * 2 + 1) * 9 + ji) * nf] = {0.0, 0.0};
+    }
+}
+
 template <int NK>
 __global__ __launch_bounds__(128, NK == 1 ? 2 : 1) void near_pairs_kernel(
     int npairs, const int2* __restrict__ list, int nf, int n0, int Q, const double* __restrict__ V,
@@ -435,69 +517,6 @@ __global__ __launch_bounds__(128, NK == 1 ? 2 : 1) void near_pairs_kernel(
         sf[a] = st[(a + 2) % 3];
     }
     const double kk[2] = {k1, k2};
-    cplx tl[NK][3][3], tk[NK][3][3];
-    for (int w = 0; w < NK; ++w)
-        for (int j = 0; j < 3; ++j)
-            for (int i = 0; i < 3; ++i) tl[w][j][i] = tk[w][j][i] = {0, 0};
-
-    // regularised double quadrature (integrandKernelsRegular) of the self pair;
-    // far_pairs_kernel has written the regular part of every other near pair
-    for (int p = 0; p < (self ? Q : 0); ++p) {
-        const double xp = qxy[2 * p], yp = qxy[2 * p + 1], wp = qw[p];
-        const double zp = 1.0 - xp - yp;
-        const V3 rp = {tt[0].x * zp + tt[1].x * xp + tt[2].x * yp, tt[0].y * zp + tt[1].y * xp + tt[2].y * yp,
-                       tt[0].z * zp + tt[1].z * xp + tt[2].z * yp};
-        for (int q = 0; q < Q; ++q) {
-            const double xq = qxy[2 * q], yq = qxy[2 * q + 1], wq = qw[q];
-            const double zq = 1.0 - xq - yq;
-            const V3 rq = {st[0].x * zq + st[1].x * xq + st[2].x * yq, st[0].y * zq + st[1].y * xq + st[2].y * yq,
-                           st[0].z * zq + st[1].z * xq + st[2].z * yq};
-            const V3 R = vsub(rp, rq);
-            const double Rm = sqrt(vdot(R, R));
-            const double cw = wp * wq;
-            cplx G[NK], Kv[NK];
-            for (int w = 0; w < NK; ++w) {
-                if (Rm < 1e-9) {
-                    G[w] = {0.0, -kk[w] * kInv4Pi};
-                    Kv[w] = {0.0, 0.0};
-                } else {
-                    const double iR = 1.0 / Rm, iR2 = iR * iR, iR3 = iR2 * iR;
-                    double s, co;
-                    sincos_moderate(kk[w] * Rm, &s, &co);
-                    G[w] = {(co - 1.0) * kInv4Pi * iR, -s * kInv4Pi * iR};
-                    // K_reg = (1/R^3 + jk/R^2) e^{-jkR} - (1/R^3 + k^2/(2R))
-                    double Kr = iR3 * co + kk[w] * iR2 * s;
-                    const double Ki = -iR3 * s + kk[w] * iR2 * co;
-                    Kr += -(iR3 + kk[w] * kk[w] * 0.5 * iR);
-                    Kv[w] = {Kr * kInv4Pi, Ki * kInv4Pi};
-                }
-            }
-            for (int j = 0; j < 3; ++j) {
-                const V3 rm = vsub(rp, tf[j]);
-                for (int i = 0; i < 3; ++i) {
-                    const V3 rn = vsub(rq, sf[i]);
-                    const double dm = 0.25 * vdot(rm, rn);
-                    const double trip = 0.25 * vdot(vcross(rm, rn), R);
-                    for (int w = 0; w < NK; ++w) {
-                        const double md = dm - 1.0 / (kk[w] * kk[w]);
-                        tl[w][j][i].re += G[w].re * md * cw;
-                        tl[w][j][i].im += G[w].im * md * cw;
-                        if (!self) {
-                            tk[w][j][i].re += Kv[w].re * trip * cw;
-                            tk[w][j][i].im += Kv[w].im * trip * cw;
-                        }
-                    }
-                }
-            }
-        }
-    }
-    for (int w = 0; w < NK; ++w)
-        for (int j = 0; j < 3; ++j)
-            for (int i = 0; i < 3; ++i) {
-                const double s = lm[j] * ln[i];
-                tl[w][j][i].re *= s; tl[w][j][i].im *= s;
-                tk[w][j][i].re *= s; tk[w][j][i].im *= s;
-            }
 
     // singular part: Wilton inner integrals, Dunavant outer rule
     const V3 nn = vunit(vcross(vsub(st[1], st[0]), vsub(st[2], st[0])));
@@ -539,7 +558,7 @@ __global__ __launch_bounds__(128, NK == 1 ? 2 : 1) void near_pairs_kernel(
                 const double s = asc * lm[j] * ln[i];
                 cplx* pa = &Wn[(size_t)((w * 2 + 0) * 9 + j * 3 + i) * nf];
                 cplx* pb = &Wn[(size_t)((w * 2 + 1) * 9 + j * 3 + i) * nf];
-                cplx a = self ? tl[w][j][i] : *pa, b = self ? tk[w][j][i] : *pb;
+                cplx a = *pa, b = *pb;
                 a.re += stl[w][j][i] * s;
                 b.re += stk[w][j][i] * s;
                 *pa = a;
@@ -1193,6 +1212,8 @@ struct Assembler {
         }
         if (nn) {
             prof().begin(PROF_NEAR_PAIRS, s);
+            self_pairs_kernel<NK><<<nc, 256, 0, s>>>(nf, n0, Q, mesh->d_V.p, mesh->d_F.p, mesh->d_len.p, quad.xy.p,
+                                                     quad.w.p, k1, k2, W.p);
             near_pairs_kernel<NK><<<(nn + 127) / 128, 128, 0, s>>>((int)nn, near.p, nf, n0, Q, mesh->d_V.p,
                                                                    mesh->d_F.p, mesh->d_len.p, area.p, quad.xy.p,
                                                                    quad.w.p, k1, k2, W.p);
