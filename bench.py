@@ -210,6 +210,85 @@ def lu_gemms(n=72048, variant="f16x3"):
     print(json.dumps({"lu_gemm_launches": launches, "n": n, "variant": variant}))
 
 
+def assembly_only(workload="gstc-spindle-72k"):
+    """The FP64 assembly of the workload's GSTC operator (its parameters,
+    mesh and rule), twice: the workload of the far-pair counter passes."""
+    import numpy as np
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "sie-gstc-2026_amd"))
+    import sie_mom_amd as sm
+    from sie_mom_amd import adjoint as ad
+    kind_mesh, size, rule, kind = WORKLOADS[workload]
+    assert kind_mesh == "spindle"
+    k0 = 2 * math.pi
+    mesh = sm.Mesh.spindle(2.0, 1.5, size)
+    o = sm.GstcOptions(frequency_hz=sm.frequency_for_k0(k0), quadrature_rule=rule)
+    rng = np.random.default_rng(20260101)
+    params = torch.from_numpy(rng.uniform(-2, 2, size=(mesh.nf, kind)) / k0).cuda()
+    rec = ad.records(params, kind, o)
+    Z = None
+    for r in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        Z = sm.ops.assemble_gstc(mesh, o, rec, out=Z)
+        torch.cuda.synchronize()
+        log(f"assembly {r}: {time.perf_counter() - t0:.3f} s (N = {Z.shape[0]})")
+    print(json.dumps({"workload": workload, "faces": mesh.nf, "unknowns": int(Z.shape[0])}))
+
+
+# FP64 VALU counters of one pass (rocprofv3 --pmc): per dispatch, summed over
+# the device's instances
+VALU_COUNTERS = ("SQ_INSTS_VALU_FMA_F64", "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VALU_ADD_F64",
+                 "SQ_INSTS_VALU_TRANS_F64", "SQ_INSTS_VALU", "GRBM_GUI_ACTIVE")
+
+
+def summarize_valu(pmc_csv, stats_csv, out_path, command):
+    """Writes the VALU summary of far_pairs_kernel: wave instructions per
+    launch by kind, the FP64 operations they execute (64 lanes; an FMA counts
+    two), and the share of the SIMDs' cycles their issue takes: a wave's VALU
+    instruction occupies its 16-lane SIMD for 4 cycles, 4 SIMDs per CU, over
+    the launch's GRBM_GUI_ACTIVE cycles (the CSV sums that counter over the
+    8 XCDs; each counts the same clock)."""
+    import csv
+    name_part = "far_pairs_kernel<1, true>"
+    per, span = {}, {}
+    for r in csv.DictReader(open(pmc_csv)):
+        if name_part in r["Kernel_Name"]:
+            d = per.setdefault(r["Dispatch_Id"], {})
+            d[r["Counter_Name"]] = d.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+            span[r["Dispatch_Id"]] = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+    if not per:
+        raise SystemExit(f"no {name_part} dispatches in {pmc_csv}")
+    mean = {c: sum(d.get(c, 0.0) for d in per.values()) / len(per) for c in VALU_COUNTERS}
+    durs = [float(r["AverageNs"]) for r in csv.DictReader(open(stats_csv)) if name_part in r["Name"]]
+    if not durs:
+        raise SystemExit(f"no {name_part} row in {stats_csv}")
+    dur_s = durs[0] * 1e-9
+    dur_pmc = sum(span.values()) / len(span) * 1e-9
+    f64 = sum(mean[c] for c in VALU_COUNTERS[:4])
+    flops = 64.0 * (2.0 * mean["SQ_INSTS_VALU_FMA_F64"] + mean["SQ_INSTS_VALU_MUL_F64"]
+                    + mean["SQ_INSTS_VALU_ADD_F64"] + mean["SQ_INSTS_VALU_TRANS_F64"])
+    n_xcd, n_simd = 8, 4 * 256
+    cycles = mean["GRBM_GUI_ACTIVE"] / n_xcd
+    with open(out_path, "w") as out:
+        out.write(f"# VALU work of {name_part} (GSTC spindle, N = 72048, rule 17: the bench's assembly, "
+                  f"run alone by bench.py --assembly)\n# command: {command}\n")
+        out.write(f"launches: {len(per)}\n")
+        for c in VALU_COUNTERS:
+            out.write(f"{c.lower()}_per_launch: {mean[c]:.6e}\n")
+        out.write(f"f64_wave_instructions_per_launch: {f64:.6e}\n")
+        out.write(f"f64_share_of_valu_instructions: {f64 / mean['SQ_INSTS_VALU']:.4f}\n")
+        out.write(f"executed_f64_flops_per_launch: {flops:.6e}\n")
+        out.write(f"avg_duration_ms (kernel trace run): {dur_s * 1e3:.3f}\n")
+        out.write(f"avg_duration_ms (counter run): {dur_pmc * 1e3:.3f}\n")
+        out.write(f"executed_f64_tflops (kernel trace duration): {flops / dur_s / 1e12:.2f} "
+                  f"(published FP64 vector peak 78.6)\n")
+        out.write(f"clock_mhz (GRBM_GUI_ACTIVE per XCD / counter-run duration): {cycles / dur_pmc / 1e6:.0f}\n")
+        out.write(f"valu_issue_share_of_simd_cycles: {4.0 * mean['SQ_INSTS_VALU'] / (n_simd * cycles):.4f}\n")
+        out.write(f"f64_issue_share_of_simd_cycles: {4.0 * f64 / (n_simd * cycles):.4f}\n")
+    print(open(out_path).read())
+
+
 def _pmc_mean_kib(csv_path, counter, name_part):
     import csv
     per = {}
@@ -408,6 +487,10 @@ def main():
         return summarize_pmc(*sys.argv[2:6], " ".join(sys.argv[6:]))
     if len(sys.argv) > 1 and sys.argv[1] == "--summarize-stats":
         return summarize_stats(sys.argv[2], sys.argv[3], sys.argv[4], " ".join(sys.argv[5:]))
+    if len(sys.argv) > 1 and sys.argv[1] == "--assembly":
+        return assembly_only()
+    if len(sys.argv) > 1 and sys.argv[1] == "--summarize-valu":
+        return summarize_valu(sys.argv[2], sys.argv[3], sys.argv[4], " ".join(sys.argv[5:]))
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
