@@ -1101,6 +1101,74 @@ __device__ __forceinline__ void read_rows_tagged(const unsigned long long* __res
     }
 }
 
+// The interchanges of rows kb..ke-1 (applied in order; step j interchanges
+// rows kb + j and sp[j] > kb + j, sp[j] < 0: none), at most L of them, as one
+// permutation composed by the NT threads of a workgroup in LDS:
+//   last(j): the last earlier step whose pivot row is kb + j;
+//   prev(j): the last earlier step with the same pivot row as step j;
+//   A(j) = the original row found at kb + j just before step j
+//        = A(last(j)) if last(j) exists, else kb + j (pointer jumping);
+// row kb + j then ends with A(prev(j)) if prev(j) exists, else p_j (step j
+// interchanges) or with A(j) (no interchange at j), and a pivot row p >= ke
+// ends with A(j) of the last step that chose it. Writes the moves (row
+// mpos[i] receives the entry of row msrc[i], i < *nmv; *nmv cleared by the
+// caller before the first barrier here). last, prv, par, fin: L words of LDS.
+template <int L, int NT>
+__device__ void compose_interchanges(const int* sp, int nL, int kb, int ke, int* last, int* prv, int* par, int* fin,
+                                     int* mpos, int* msrc, int* nmv) {
+    const int tid = threadIdx.x;
+    for (int j = tid; j < L; j += NT) {
+        last[j] = -1;
+        fin[j] = 1;
+    }
+    __syncthreads();
+    for (int j = tid; j < nL; j += NT) {
+        const int p = sp[j];
+        int pr = -1;
+        if (p >= 0) {
+            if (p < ke) atomicMax(&last[p - kb], j);
+            for (int i = 0; i < j; ++i)
+                if (sp[i] == p) pr = i;
+        }
+        prv[j] = pr;
+    }
+    __syncthreads();
+    for (int j = tid; j < nL; j += NT) {
+        par[j] = last[j] >= 0 ? last[j] : j;
+        if (prv[j] >= 0) fin[prv[j]] = 0;  // a later step chose the same pivot row
+    }
+    __syncthreads();
+    for (int span = 1; span < L; span *= 2) {
+        int np[(L + NT - 1) / NT];
+#pragma unroll
+        for (int u = 0; u < (L + NT - 1) / NT; ++u) {
+            const int j = tid + NT * u;
+            np[u] = j < nL ? par[par[j]] : 0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < (L + NT - 1) / NT; ++u) {
+            const int j = tid + NT * u;
+            if (j < nL) par[j] = np[u];
+        }
+        __syncthreads();
+    }
+    for (int j = tid; j < nL; j += NT) {
+        const int p = sp[j];
+        const int src = p < 0 ? kb + par[j] : prv[j] >= 0 ? kb + par[prv[j]] : p;
+        if (src != kb + j) {
+            const int t = atomicAdd(nmv, 1);
+            mpos[t] = kb + j;
+            msrc[t] = src;
+        }
+        if (p >= ke && fin[j]) {
+            const int t = atomicAdd(nmv, 1);
+            mpos[t] = p;
+            msrc[t] = kb + par[j];
+        }
+    }
+}
+
 // The FP32 panel with the candidates as keys: per column, each wave finds its
 // candidate (DPP maximum), the workgroup's candidate is the best of its waves;
 // each workgroup publishes its candidate row (tagged words), then its key into
@@ -1121,7 +1189,8 @@ __global__ __launch_bounds__(BS) void panel_getf2_keyed_kernel(int M, int w, cx<
                                                                unsigned long long* __restrict__ pub,
                                                                unsigned long long* __restrict__ drow,
                                                                unsigned tag_base, int* __restrict__ info,
-                                                               int info_base, WaitArg wa) {
+                                                               int info_base, WaitArg wa, int sc0, int snc, int sc1,
+                                                               int snc1) {
     static_assert(W % 8 == 0 && W <= 32, "columns are handled in groups of 8, a row read by one wave");
     constexpr int kStalled = -2;  // swin code: the key exchange gave up (-1: singular column)
     constexpr int NG = W / 8;
@@ -1143,6 +1212,7 @@ __global__ __launch_bounds__(BS) void panel_getf2_keyed_kernel(int M, int w, cx<
     __shared__ cx<float> sprow[W];
     __shared__ cx<float> sdrow[W];
     __shared__ float stage_c[2 * W], stage_j[2 * W];  // rows on their way out
+    __shared__ int spv[W];  // the pivot row of each column (-1: no interchange)
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 
     for (int j = 0; j < w; ++j) {
@@ -1244,10 +1314,12 @@ __global__ __launch_bounds__(BS) void panel_getf2_keyed_kernel(int M, int w, cx<
                 ipiv[j] = j + ipiv_off;
                 if (*info == 0) *info = info_base + j + 1;
             }
+            if (threadIdx.x == 0) spv[j] = -1;
             continue;
         }
         const int p = code & 0xFFFFF;
         if (blockIdx.x == 0 && threadIdx.x == 0) ipiv[j] = p + ipiv_off;
+        if (threadIdx.x == 0) spv[j] = p > j ? p : -1;
         // swap: position p takes old row j, position j takes the pivot row
         if (own && r == p && p != j) {
 #pragma unroll
@@ -1302,6 +1374,30 @@ __global__ __launch_bounds__(BS) void panel_getf2_keyed_kernel(int M, int w, cx<
         for (int k = 0; k < W; ++k)
             if (k < w) P[(long long)k * lda + r] = a[k];
     }
+    // The panel's interchanges on the other columns of its NB block: column
+    // offsets (from P) sc0 .. sc0 + snc - 1 and sc1 .. sc1 + snc1 - 1, rows of
+    // the panel (local 0 .. M-1). Every workgroup composes them (spv is the
+    // same in all) and takes every G-th group of BS/64 columns, one column
+    // per wave, one move per lane: all of a column's moved entries are loaded
+    // before any is stored. The panel's own columns are not among them.
+    const int ns = snc + snc1;
+    if (ns > 0) {
+        __shared__ int last[W], prv[W], par[W], fin[W], mpos[2 * W], msrc[2 * W];
+        __shared__ int nmv;
+        if (threadIdx.x == 0) nmv = 0;
+        compose_interchanges<W, BS>(spv, w, 0, w, last, prv, par, fin, mpos, msrc, &nmv);
+        __syncthreads();
+        const int nm = nmv;  // <= 2W = 64: one lane each
+        for (int t0 = blockIdx.x * (BS / 64); t0 < ns; t0 += G * (BS / 64)) {
+            const int t = t0 + wv;
+            const bool live = t < ns && lane < nm;
+            cx<float>* col = P + (long long)(t < snc ? sc0 + t : sc1 + (t - snc)) * lda;
+            cx<float> v = {0, 0};
+            if (live) v = col[msrc[lane]];
+            __builtin_amdgcn_wave_barrier();  // one wave owns the column: its loads precede its stores
+            if (live) col[mpos[lane]] = v;
+        }
+    }
 }
 
 // ============================================================== row swaps
@@ -1329,19 +1425,11 @@ __global__ void laswp_kernel(cx<T>* __restrict__ A, long long lda, int rows, int
     }
 }
 
-// The interchanges of rows kb..ke-1 (ipiv, applied in order, p > k) as one
-// permutation, for at most L interchanges. Every workgroup composes it in LDS
-// (step j interchanges rows k = kb + j and p_j):
-//   last(j): the last earlier step whose pivot row is kb + j;
-//   prev(j): the last earlier step with the same pivot row as step j;
-//   A(j) = the original row found at kb + j just before step j
-//        = A(last(j)) if last(j) exists, else kb + j (pointer jumping);
-// row kb + j then ends with A(prev(j)) if prev(j) exists, else p_j (step j
-// interchanges) or with A(j) (no interchange at j), and a pivot row p >= ke
-// ends with A(j) of the last step that chose it. Then each column's moved
-// entries are all loaded before any is stored. cols: columns per workgroup.
-// reverse: the interchanges in the opposite order (the inverse permutation:
-// every move taken backwards).
+// The interchanges ipiv[kb..ke) (p > k), at most L, applied as one permutation
+// (compose_interchanges, in every workgroup) to the columns [c0, c0+nc) and
+// [c1, c1+nc1): each column's moved entries are all loaded before any is
+// stored. cols: columns per workgroup. reverse: the interchanges in the
+// opposite order (the inverse permutation: every move taken backwards).
 template <typename T, int L>
 __global__ __launch_bounds__(256) void laswp_perm_kernel(cx<T>* __restrict__ A, long long lda, int rows, int c0,
                                                          int nc, int c1, int nc1, const int* __restrict__ ipiv,
@@ -1364,55 +1452,8 @@ __global__ __launch_bounds__(256) void laswp_perm_kernel(cx<T>* __restrict__ A, 
             if (!(p > kb + j && p < rows)) p = -1;
         }
         sp[j] = p;
-        last[j] = -1;
-        fin[j] = 1;
     }
-    __syncthreads();
-    for (int j = tid; j < nL; j += 256) {
-        const int p = sp[j];
-        int pr = -1;
-        if (p >= 0) {
-            if (p < ke) atomicMax(&last[p - kb], j);
-            for (int i = 0; i < j; ++i)
-                if (sp[i] == p) pr = i;
-        }
-        prv[j] = pr;
-    }
-    __syncthreads();
-    for (int j = tid; j < nL; j += 256) {
-        par[j] = last[j] >= 0 ? last[j] : j;
-        if (prv[j] >= 0) fin[prv[j]] = 0;  // a later step chose the same pivot row
-    }
-    __syncthreads();
-    for (int span = 1; span < L; span *= 2) {
-        int np[(L + 255) / 256];
-#pragma unroll
-        for (int u = 0; u < (L + 255) / 256; ++u) {
-            const int j = tid + 256 * u;
-            np[u] = j < nL ? par[par[j]] : 0;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int u = 0; u < (L + 255) / 256; ++u) {
-            const int j = tid + 256 * u;
-            if (j < nL) par[j] = np[u];
-        }
-        __syncthreads();
-    }
-    for (int j = tid; j < nL; j += 256) {
-        const int p = sp[j];
-        const int src = p < 0 ? kb + par[j] : prv[j] >= 0 ? kb + par[prv[j]] : p;
-        if (src != kb + j) {
-            const int t = atomicAdd(&nmv, 1);
-            mpos[t] = kb + j;
-            msrc[t] = src;
-        }
-        if (p >= ke && fin[j]) {
-            const int t = atomicAdd(&nmv, 1);
-            mpos[t] = p;
-            msrc[t] = kb + par[j];
-        }
-    }
+    compose_interchanges<L, 256>(sp, nL, kb, ke, last, prv, par, fin, mpos, msrc, &nmv);
     __syncthreads();
     const int nm = nmv;
     const int* from = reverse ? mpos : msrc;
@@ -2554,10 +2595,13 @@ static void panel_begin(int n, hipStream_t s) {
 }
 
 // Factors the M x w panel at P; ipiv[j] receives the absolute row (panel row
-// + ipiv_off) swapped with row j.
+// + ipiv_off) swapped with row j. The FP32 (keyed) panel also applies its
+// interchanges to the columns at offsets [sc0, sc0 + snc) and [w, w + snc1)
+// from P (the rest of its NB block) and returns true; the FP64 panel leaves
+// them to the caller (false).
 template <typename T>
-static void panel(int M, int w, cx<T>* P, long long lda, int* ipiv, int ipiv_off, int* d_info, int info_base,
-                  hipStream_t s) {
+static bool panel(int M, int w, cx<T>* P, long long lda, int* ipiv, int ipiv_off, int* d_info, int info_base,
+                  hipStream_t s, int sc0 = 0, int snc = 0, int snc1 = 0) {
     constexpr int W = PanelCfg<T>::W;
     constexpr bool keyed = std::is_same<T, float>::value;
     const void* kern = keyed ? (const void*)panel_getf2_keyed_kernel<W, kPanelBS>
@@ -2600,8 +2644,9 @@ static void panel(int M, int w, cx<T>* P, long long lda, int* ipiv, int ipiv_off
         unsigned long long* pubw = (unsigned long long*)pub;
         unsigned long long* droww = (unsigned long long*)drow;
         WaitArg wa = wait_arg();
-        void* args[] = {&M, &w, &P, &lda, &ipiv, &ipiv_off, &kcur, &knext, &nkeys, &pubw, &droww, &tag_base,
-                        &d_info, &info_base, &wa};
+        int sc1 = w;
+        void* args[] = {&M,       &w,         &P,      &lda,       &ipiv, &ipiv_off, &kcur, &knext, &nkeys, &pubw,
+                        &droww,   &tag_base,  &d_info, &info_base, &wa,   &sc0,      &snc,  &sc1,   &snc1};
         if (g_cooperative_panels) SIE_HIP(hipLaunchCooperativeKernel(kern, dim3(G), dim3(kPanelBS), args, 0, s));
         else SIE_HIP(hipLaunchKernel(kern, dim3(G), dim3(kPanelBS), args, 0, s));
     } else {
@@ -2614,6 +2659,7 @@ static void panel(int M, int w, cx<T>* P, long long lda, int* ipiv, int ipiv_off
         else SIE_HIP(hipLaunchKernel(kern, dim3(G), dim3(kPanelBS), args, 0, s));
     }
     prof().end(PROF_PANEL, s, 8.0 * M * w * w / 2, 2.0 * sizeof(cx<T>) * M * w);
+    return keyed;
 }
 
 // block of the right-looking steps of the innermost level (NB = 512 measured
@@ -2698,8 +2744,8 @@ static void getrf_block(int n, cx<T>* A, long long lda, int c0, int c1, int* ipi
         for (int k1 = k0; k1 < k0 + nb; k1 += W) {
             const int w = std::min(W, k0 + nb - k1);
             cx<T>* P = A + (long long)k1 * lda + k1;
-            panel<T>(n - k1, w, P, lda, ipiv + k1, k1, d_info, k1, s);
-            laswp2<T>(A, lda, n, k0, k1 - k0, k1 + w, k0 + nb - (k1 + w), ipiv, k1, k1 + w, s);
+            if (!panel<T>(n - k1, w, P, lda, ipiv + k1, k1, d_info, k1, s, k0 - k1, k1 - k0, k0 + nb - (k1 + w)))
+                laswp2<T>(A, lda, n, k0, k1 - k0, k1 + w, k0 + nb - (k1 + w), ipiv, k1, k1 + w, s);
             const int rest = k0 + nb - (k1 + w);
             if (rest > 0) {
                 trsm_small<T>(w, P, lda, A + (long long)(k1 + w) * lda + k1, lda, rest, 0, s);
